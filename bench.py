@@ -1,0 +1,165 @@
+"""Benchmark: sieved numbers/s of the plain [1, N] pipeline at N = 1e11
+(SEG_NUM = 3e10) on one MI355X, with Miller-Rabin tests/s beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One step = one full voss_run_plain(N=1e11, SEG_NUM=3e10): planning, the
+sieve+extract+histogram kernel, the stitch kernel, the copy of the
+accumulators and the host finalisation (the reference's TOTAL TIME window,
+src/voss_v7_live.cu:432 -> 632).  The input is the number N itself, so
+nothing crosses PCIe before the timed region.
+
+Prints ONE JSON line (rank 0).  `roofline` describes the dominant kernel
+(sieve_kernel): algorithmic LDS bytes per launch (8 bytes per dword
+read-modify-write: one per (prime, word) for the presieved primes below 32
+and one per mark for the rest, plus the bitmap written once and read once)
+over its average duration from HIP events on the library's stream.
+`cpu_baseline` times the CPU oracle (a port of the reference semantics) on
+a bounded sample, [1, 1e10], on the host cores.
+"""
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "voss-prime-gaps_v1_amd")
+sys.path.insert(0, ROOT)
+sys.path.insert(0, PKG)
+
+import numpy as np  # noqa: E402
+
+N_BENCH = 10**11
+SEG_NUM = 30000000000
+CPU_SAMPLE_N = 10**10
+
+# MI355X LDS: 160 KB per CU, 128 bytes/clock per CU, 256 CUs at 2.4 GHz.
+LDS_PEAK_GBS = 128 * 256 * 2.4e9 / 1e9
+
+
+def model_lds_bytes(N):
+    """Algorithmic LDS bytes of the sieve for [1, N] (see module docstring).
+
+    Marks: for every base prime 7 <= p <= sqrt(N), the multiples p*m, m >= p,
+    m coprime to 30, that are <= N: sum_p (N/p - p) * 8/30 (plus rounding).
+    Primes below 32 are charged one dword RMW per (prime, word) instead."""
+    limit = int(math.isqrt(N)) + 1
+    s = np.ones(limit + 1, dtype=bool)
+    s[:2] = False
+    for i in range(2, int(limit**0.5) + 1):
+        if s[i]:
+            s[i * i::i] = False
+    ps = np.nonzero(s)[0]
+    ps = ps[ps >= 7].astype(np.float64)
+    words = N / 120.0
+    small = ps[ps < 32]
+    big = ps[ps >= 32]
+    marks_big = float(np.sum(np.maximum(N / big - big, 0) * 8.0 / 30.0))
+    m_eff = len(small) * words + marks_big
+    return 8.0 * m_eff + 2.0 * (N / 30.0), m_eff
+
+
+def cpu_baseline(threads):
+    from oracle import oracle as O
+    t = time.perf_counter()
+    r = O.run_plain(CPU_SAMPLE_N, SEG_NUM, threads=threads)
+    dt = time.perf_counter() - t
+    assert r["total_primes"] == 455052511
+    return {"value": CPU_SAMPLE_N / dt, "unit": "numbers/s", "cores": threads,
+            "kind": "port", "seconds": round(dt, 3),
+            "sample": f"plain [1, {CPU_SAMPLE_N:.0e}] SEG_NUM=3e10, oracle C/OpenMP, "
+                      f"{threads} threads"}
+
+
+def mr_candidates(eng, count, seed=0x564F5353 + 2):
+    """Half odd uniform in [2^40, 2^64), half gap endpoints of a sieve run."""
+    rng = np.random.default_rng(seed)
+    half = count // 2
+    rnd = rng.integers(2**40, 2**64 - 1, size=count - half, dtype=np.uint64,
+                       endpoint=False) | np.uint64(1)
+    r = eng.run_interval(10**15, 10**15 + 2 * 10**10, large_cap=1 << 16)
+    ends = np.array([p for p, g, _ in r["large"]] + [p - g for p, g, _ in r["large"]],
+                    dtype=np.uint64)
+    pick = ends[rng.integers(0, len(ends), size=half)]
+    c = np.concatenate([rnd, pick])
+    rng.shuffle(c)
+    return c
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mr-count", type=int, default=10**7)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sieve would shard by range with one tiny histogram all-reduce)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import vossgpu
+
+    eng = vossgpu.Engine(0)
+    for _ in range(args.warmup):
+        eng.run_plain(N_BENCH, SEG_NUM)
+    step_s, sieve_ms, dev_ms = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        r = eng.run_plain(N_BENCH, SEG_NUM)
+        step_s.append(time.perf_counter() - ts)
+        sieve_ms.append(r["sieve_ms"])
+        dev_ms.append(r["device_ms"])
+    total = time.perf_counter() - t0
+    assert r["total_primes"] == 4118054813, r["total_primes"]
+
+    lds_bytes, m_eff = model_lds_bytes(N_BENCH)
+    avg_sieve_s = statistics.mean(sieve_ms) / 1e3
+    achieved = lds_bytes / avg_sieve_s / 1e9
+
+    # Miller-Rabin tests/s on device-resident candidates
+    cand = mr_candidates(eng, args.mr_count)
+    verdict, mr_ms = eng.mr_device(cand, repeats=3)
+    mr_rate = len(cand) / (statistics.median(mr_ms) / 1e3)
+
+    out = {
+        "metric": "sieved numbers/s at N=1e11 (plain [1,N], SEG_NUM=3e10, bit-exact outputs)",
+        "value": N_BENCH * args.steps / total,
+        "unit": "numbers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/u64 integer",
+        "data": "synthetic (the range [1, 1e11] itself; MR candidates seeded SplitMix-free "
+                "numpy PCG64 draws + sieve gap endpoints)",
+        "config": {"workload": "plain sieve+gaps [1, 1e11], SEG_NUM=3e10 (4 logical segments), "
+                               "131072-byte LDS windows", "N": N_BENCH, "SEG_NUM": SEG_NUM,
+                   "windows": r["windows"], "device": eng.name},
+        "roofline": {"bound": "lds", "kernel": "sieve_kernel",
+                     "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / LDS_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": lds_bytes, "m_eff": m_eff,
+                     "avg_kernel_ms": statistics.mean(sieve_ms)},
+        "device_ms_per_step": statistics.mean(dev_ms),
+        "mr": {"tests_per_s": mr_rate, "count": int(len(cand)),
+               "primes": int(verdict.sum()), "kernel_ms": mr_ms},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(threads)
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

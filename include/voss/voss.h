@@ -1,0 +1,168 @@
+/*
+ * voss.h -- C ABI of the MI355X (gfx950) VOSS prime-gap engine.
+ *
+ * The reference (src/voss_v7_live.cu, src/verify_cpp.cpp in the reference
+ * tree) has no library API: its boundary is a process boundary (constants
+ * patched into the CUDA source, files written into cwd, "Total primes:" on
+ * stdout, exit codes).  This header is the in-process boundary underneath
+ * that process boundary.  The CLI binaries voss_v7_live and verify_cpp in
+ * this package are thin C++ programs over it, and src/voss_master.py drives
+ * them exactly as the reference's run_voss.py expects.  Each entry point
+ * names the reference code it replaces.
+ *
+ * Conventions
+ *   - All functions return VOSS_OK (0) or a negative VOSS_E* code; none of
+ *     them calls exit().  voss_last_error() gives the message of the last
+ *     failure on the calling thread.
+ *   - Plain pointers and sizes only.  Output buffers are caller-owned.
+ *   - A voss_ctx owns device memory, streams and cached base-prime tables
+ *     for one GPU.  A context is used by one host thread at a time.
+ *   - Integers only on the device; merit is computed on the host with the C
+ *     library's log, as float((double)gap / log((double)p_before)).
+ */
+#ifndef VOSS_VOSS_H
+#define VOSS_VOSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VOSS_OK 0
+#define VOSS_EINVAL (-1)  /* bad argument (range, SEG_NUM % 30, ...)        */
+#define VOSS_EDEVICE (-2) /* HIP runtime / kernel failure                   */
+#define VOSS_ENOMEM (-3)  /* device or host allocation failed               */
+#define VOSS_ERANGE (-4)  /* a gap beyond the device histogram (never < 2^64) */
+#define VOSS_EBUSY (-5)   /* serving queue full / HBM budget exhausted      */
+#define VOSS_ENOTFOUND (-6) /* unknown serving request id                   */
+
+/* Limits of the reference's output arrays (src/voss_v7_live.cu:23-35). */
+#define VOSS_MAX_GAP 100000
+#define VOSS_MODQ_MAX_GAP 100
+#define VOSS_MODQ_ROWS 10 /* q=6 residues {1,5}, then q=30 residues {1,7,...,29} */
+#define VOSS_Q_SLOTS 40   /* q=6 at slots 0..5, q=30 at slots 6..35 */
+#define VOSS_GAP_SEQ_SIZE 5000000
+
+/* struct LargeGap (src/voss_v7_live.cu:40): position is p_after. */
+typedef struct {
+    uint64_t position;
+    int32_t gap;
+    float merit;
+} voss_large_gap;
+
+/* Result of one sieve run.  Plain mode follows the reference's conventions
+ * exactly (total_primes includes 2, 3, 5; c1/c3/modq count primes >= 7;
+ * gaps[1] and gaps[2] include (2,3) and (3,5); modq_gap and large omit
+ * pairs that straddle a SEG_NUM boundary).  Interval mode counts every
+ * prime of [lo, hi) by its true residue and every consecutive pair inside
+ * the interval. */
+typedef struct {
+    uint64_t total_primes;
+    uint64_t first_prime; /* 0 if the range holds no prime */
+    uint64_t last_prime;  /* plain mode: 5 when N < 7, like the reference */
+    uint64_t c1, c3;      /* primes with p%4==1 / p%4==3 (see above) */
+    uint64_t modq[VOSS_Q_SLOTS];
+    int64_t gaps[VOSS_MAX_GAP];
+    int64_t modq_gap[VOSS_MODQ_ROWS * VOSS_MODQ_MAX_GAP];
+    uint64_t large_count;    /* out: untruncated number of large gaps */
+    uint64_t large_capacity; /* in: entries available at `large` */
+    voss_large_gap* large;   /* in: caller buffer or NULL; out: sorted by
+                                merit desc, then position asc */
+    double seconds;          /* out: wall time from planning to the final
+                                histogram on the host (src/voss_v7_live.cu:
+                                432 -> 632 window) */
+    double device_ms;        /* out: summed device time of the sieve and
+                                stitch kernels (HIP events) */
+    double sieve_ms;         /* out: device time of the sieve kernel alone */
+    uint64_t windows;        /* out: LDS windows sieved */
+} voss_result;
+
+typedef struct voss_ctx voss_ctx;
+
+/* Tunables chosen by voss_autotune() or by hand; zero fields keep the
+ * built-in defaults. */
+typedef struct {
+    uint32_t window_bytes;  /* wheel bytes per LDS window (multiple of 4096) */
+    uint32_t wg_per_cu;     /* resident workgroups per CU in the sieve grid */
+    uint32_t tier_a_limit;  /* primes below it mark with a whole wave each */
+    uint32_t reserved;
+} voss_tuning;
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the reference's per-run setup in main(): cudaGetDeviceProperties,
+ * cudaStreamCreate and the cudaMalloc block (src/voss_v7_live.cu:353-401). */
+int voss_create(int device, voss_ctx** out);
+void voss_destroy(voss_ctx* ctx);
+const char* voss_last_error(void);
+int voss_device_name(voss_ctx* ctx, char* buf, size_t cap, int* num_cus,
+                     double* total_gb);
+int voss_set_tuning(voss_ctx* ctx, const voss_tuning* t);
+int voss_get_tuning(voss_ctx* ctx, voss_tuning* t);
+
+/* ------------------------------------------------------------------ sieve */
+
+/* Plain [1, N] run with logical SEG_NUM segments: the whole hot loop of the
+ * reference's main() -- per-segment planning and sieve_w30_seg_kernel
+ * (:434-502), extract_w30_seg_kernel + thrust::sort (:506-524), the host
+ * boundary gap (:551-562), mod4_count_kernel / modq_count_kernel (:564-570),
+ * gaps_w30_seg_kernel (:574-579), the host reduce (:581-594) and the final
+ * +1 at g=1,2 (:634-635).  Requires N >= 5 and seg_num % 30 == 0, like the
+ * reference (:340-342, :404). */
+int voss_run_plain(voss_ctx* ctx, uint64_t N, uint64_t seg_num,
+                   voss_result* out);
+
+/* gap_sequence sample of a plain run (sample_gaps_kernel and its host logic,
+ * src/voss_v7_live.cu:235-253, :526-547): gap_seq/pos_seq hold
+ * VOSS_GAP_SEQ_SIZE entries each, zero where the reference writes nothing.
+ * rows_out = number of rows with gap > 0. */
+int voss_sample_plain(voss_ctx* ctx, uint64_t N, uint64_t seg_num,
+                      int32_t* gap_seq, uint64_t* pos_seq, uint64_t* rows_out);
+
+/* Interval [lo, hi) run.  No reference counterpart (the reference sieves
+ * only [1, N]); it is the plain pipeline applied to an offset interval. */
+int voss_run_interval(voss_ctx* ctx, uint64_t lo, uint64_t hi,
+                      voss_result* out);
+
+/* Many intervals in one persistent launch (captured in a HIP graph).
+ * out[i] receives interval i; out[i].large/large_capacity are honoured. */
+int voss_run_batch(voss_ctx* ctx, const uint64_t* lo, const uint64_t* hi,
+                   size_t n, voss_result* out);
+
+/* The same intervals with one launch sequence per interval (the plain
+ * per-interval form the batched form is measured against). */
+int voss_run_batch_plain(voss_ctx* ctx, const uint64_t* lo,
+                         const uint64_t* hi, size_t n, voss_result* out);
+
+/* ----------------------------------------------------------- Miller-Rabin */
+
+/* Deterministic 12-base Miller-Rabin on the GPU (64-bit Montgomery form):
+ * replaces is_prime_u64 / mod_pow / mul_mod (src/verify_cpp.cpp:24-57).
+ * verdict[i] = 1 iff n[i] is prime.  Host buffers. */
+int voss_mr_batch(voss_ctx* ctx, const uint64_t* n, size_t count,
+                  uint8_t* verdict);
+
+/* Same on device-resident buffers (allocated with voss_device_alloc);
+ * kernel_ms (optional) receives the device time of the MR kernels. */
+int voss_mr_batch_device(voss_ctx* ctx, const uint64_t* d_n, size_t count,
+                         uint8_t* d_verdict, double* kernel_ms);
+
+/* verify_gap (src/verify_cpp.cpp:59-93) for a batch of gaps: both ends
+ * prime (GPU Miller-Rabin) and every interior number composite (GPU
+ * interval sieve).  ok[i] = 1 iff gap i verifies. */
+int voss_verify_gaps(voss_ctx* ctx, const uint64_t* p_after,
+                     const int32_t* gap, size_t count, uint8_t* ok);
+
+/* --------------------------------------------------- device memory helpers */
+
+int voss_device_alloc(voss_ctx* ctx, size_t bytes, void** dptr);
+int voss_device_free(voss_ctx* ctx, void* dptr);
+int voss_memcpy_h2d(voss_ctx* ctx, void* dst, const void* src, size_t bytes);
+int voss_memcpy_d2h(voss_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

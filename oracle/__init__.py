@@ -1,0 +1,5 @@
+"""CPU oracle for the VOSS hot path -- test infrastructure only.
+
+Imported only by tests/, by ``__graft_entry__.smoke()`` (as the checker) and
+by ``bench.py``'s cpu_baseline leg.  The product never imports it.
+"""

@@ -1,0 +1,46 @@
+"""GPU Miller-Rabin and gap verification parity against the oracle
+(is_prime_u64 / verify_gap restatements, pinned by the reference binary)."""
+import random
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KAT = [0, 1, 2, 3, 4, 5, 9, 25, 37, 41, 561, 1105, 1729, 2047, 41041, 1373653, 25326001,
+       3215031751, 2152302898747, 3474749660383, 341550071728321, 3825123056546413051,
+       2**61 - 1, 2**64 - 59, 2**64 - 1, 2**63 + 1, 2**62 + 135, 2**31 - 1, 1000000007]
+
+
+def test_mr_kats(engine, oracle):
+    v = engine.mr(np.array(KAT, dtype=np.uint64))
+    assert [int(x) for x in v] == [int(oracle.is_prime(n)) for n in KAT]
+
+
+def test_mr_small_exhaustive(engine, oracle):
+    ns = np.arange(0, 3 * 10**6, dtype=np.uint64)
+    assert np.array_equal(engine.mr(ns), oracle.is_prime_batch(ns))
+
+
+def test_mr_random_64bit(engine, oracle):
+    rng = np.random.default_rng(0x564F5353 + 2)
+    ns = rng.integers(0, 2**63, size=10**6, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    assert np.array_equal(engine.mr(ns), oracle.is_prime_batch(ns))
+
+
+def test_mr_device_path(engine, oracle):
+    rng = np.random.default_rng(7)
+    ns = rng.integers(2**40, 2**63, size=200000, dtype=np.uint64) | np.uint64(1)
+    v, times = engine.mr_device(ns, repeats=2)
+    assert np.array_equal(v, oracle.is_prime_batch(ns))
+    assert len(times) == 2 and all(t > 0 for t in times)
+
+
+def test_verify_gaps(engine, oracle):
+    r = oracle.run_plain(10**9, 30000000000)
+    rows = [(p, g) for p, g, _ in r["large"]]
+    rows += [(p, g - 2) for p, g in rows[:40]] + [(p + 2, g + 2) for p, g in rows[:40]]
+    rows += [(11, 4), (13, 6), (7, 0), (15, 4), (9, 1), (25056082543, 456), (25056082543, 454)]
+    pa = np.array([p for p, _ in rows], dtype=np.uint64)
+    g = np.array([x for _, x in rows], dtype=np.int32)
+    assert np.array_equal(engine.verify_gaps(pa, g), oracle.verify_gaps(pa, g))

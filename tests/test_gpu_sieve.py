@@ -1,0 +1,148 @@
+"""GPU sieve parity: the gfx950 library (through the C ABI) against the CPU
+oracle on the same ranges, and against the committed 1e11 fixtures.
+Integer outputs are compared bit-exactly; merits are the same float32."""
+import json
+import math
+import os
+import random
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("total_primes", "first_prime", "last_prime", "c1", "c3", "large_count")
+
+
+def assert_same(got, ref, large=True):
+    for k in FIELDS:
+        assert got[k] == ref[k], (k, got[k], ref[k])
+    assert np.array_equal(got["modq"], ref["modq"])
+    assert np.array_equal(got["gaps"], ref["gaps"])
+    assert np.array_equal(got["modq_gap"], ref["modq_gap"])
+    if large:
+        assert got["large"] == [(p, g, float(np.float32(m))) for p, g, m in ref["large"]]
+
+
+@pytest.mark.parametrize("N", [5, 6, 7, 10, 30, 31, 100, 1000, 12345, 10**6, 10**7 + 19,
+                               10**8, 10**9])
+def test_plain_matches_oracle(engine, oracle, N):
+    print(f"[gpu] plain N={N}", flush=True)
+    assert_same(engine.run_plain(N, 30000000000), oracle.run_plain(N, 30000000000))
+
+
+@pytest.mark.parametrize("N,SEG", [(10**6, 30030), (10**8, 3000000), (2 * 10**7, 4000020),
+                                   (10**5, 30), (10**9, 30000000), (123456789, 3932160)])
+def test_plain_segment_rule_matches_oracle(engine, oracle, N, SEG):
+    print(f"[gpu] plain N={N} SEG_NUM={SEG}", flush=True)
+    assert_same(engine.run_plain(N, SEG), oracle.run_plain(N, SEG))
+
+
+def test_plain_1e11_fixture(engine):
+    """N = 1e11, SEG_NUM = 3e10: the reference's histogram snapshot and the
+    oracle fixture for every other output."""
+    print("[gpu] plain N=1e11", flush=True)
+    fx = json.load(open(os.path.join(GOLDEN, "oracle_plain_1e11.json")))
+    r = engine.run_plain(10**11, 30000000000, large_cap=1 << 20)
+    for k in FIELDS:
+        assert r[k] == fx[k], k
+    assert [int(x) for x in r["modq"]] == fx["modq"]
+    assert {str(g): int(c) for g, c in enumerate(r["gaps"]) if c} == fx["gaps"]
+    assert {f"{row},{g}": int(c) for row in range(10) for g, c in enumerate(r["modq_gap"][row])
+            if c} == fx["modq_gap"]
+    import hashlib
+    h = hashlib.sha256()
+    for p, g, m in r["large"]:
+        h.update(f"{p},{g},{m:.6f}\n".encode())
+    assert h.hexdigest() == fx["large_sha256"]
+    # and the reference's own snapshot of the histogram at 1e11
+    import csv
+    with open(os.path.join(GOLDEN, "reference", "hist_100000000000.csv")) as f:
+        ref = {int(a): int(b) for a, b in list(csv.reader(f))[1:]}
+    assert {g: int(c) for g, c in enumerate(r["gaps"]) if c} == ref
+
+
+@pytest.mark.parametrize("window", [65536, 32768])
+def test_window_sizes_agree(engine, oracle, window):
+    print(f"[gpu] window={window}", flush=True)
+    ref = oracle.run_plain(10**9 + 7, 300000000)
+    old = engine.tuning()
+    try:
+        engine.set_tuning(window_bytes=window, wg_per_cu={65536: 2, 32768: 3}[window])
+        assert_same(engine.run_plain(10**9 + 7, 300000000), ref)
+    finally:
+        engine.set_tuning(**old)
+
+
+def test_tier_split_agrees(engine, oracle):
+    ref = oracle.run_plain(10**8, 30000000000)
+    old = engine.tuning()
+    try:
+        for lim in (80, 4096, 200000):
+            engine.set_tuning(tier_a_limit=lim)
+            assert_same(engine.run_plain(10**8, 30000000000), ref)
+    finally:
+        engine.set_tuning(**old)
+
+
+INTERVALS = [(0, 100), (1, 2), (2, 3), (2, 4), (3, 8), (5, 6), (7, 7), (7, 8), (0, 10**6),
+             (10**9, 10**9 + 10**7), (10**12, 10**12 + 10**6), (2**40, 2**40 + 5 * 10**6),
+             (10**15, 10**15 + 10**5), (10**16 - 10**5, 10**16 + 10**5),
+             (25056082000, 25056083000)]
+
+
+@pytest.mark.parametrize("lo,hi", INTERVALS)
+def test_interval_matches_oracle(engine, oracle, lo, hi):
+    print(f"[gpu] interval [{lo},{hi})", flush=True)
+    assert_same(engine.run_interval(lo, hi), oracle.run_interval(lo, hi))
+
+
+def seeded_intervals(n, seed, lo_min=10**9, lo_max=10**13, lmin=3, lmax=7):
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        lo = rng.randrange(lo_min, lo_max)
+        ln = int(10 ** (lmin + (lmax - lmin) * rng.random()))
+        out.append((lo, lo + ln))
+    return out
+
+
+def test_batch_matches_single_and_oracle(engine, oracle):
+    iv = seeded_intervals(48, 0x564F5353 + 1)
+    iv += [(0, 1000), (10**6, 10**6), (7, 8)]
+    lo = [a for a, _ in iv]
+    hi = [b for _, b in iv]
+    print(f"[gpu] batch of {len(iv)}", flush=True)
+    batch = engine.run_batch(lo, hi, large_cap=4096)
+    single = engine.run_batch_plain(lo, hi, large_cap=4096)
+    for (a, b), rb, rs in zip(iv, batch, single):
+        ref = oracle.run_interval(a, b)
+        assert_same(rb, ref)
+        assert_same(rs, ref)
+
+
+def test_invariants_interval_large(engine):
+    """Size-independent checks at a config-3-like offset: sum N(g) = pi - 1,
+    sum g N(g) = p_last - p_first, Chebyshev split adds up."""
+    lo, hi = 10**13, 10**13 + 10**10
+    print(f"[gpu] interval [{lo},{hi}) invariants", flush=True)
+    r = engine.run_interval(lo, hi)
+    g = r["gaps"]
+    assert int(g.sum()) == r["total_primes"] - 1
+    assert int((np.arange(len(g), dtype=np.int64) * g).sum()) == r["last_prime"] - r["first_prime"]
+    assert r["c1"] + r["c3"] == r["total_primes"]
+    # prime number theorem sanity: within 1% of li-density estimate
+    est = (hi - lo) / math.log((lo + hi) / 2)
+    assert abs(r["total_primes"] / est - 1) < 0.01
+
+
+def test_bad_arguments(engine):
+    import vossgpu
+    with pytest.raises(vossgpu.VossError):
+        engine.run_plain(4, 30)
+    with pytest.raises(vossgpu.VossError):
+        engine.run_plain(1000, 31)
+    with pytest.raises(vossgpu.VossError):
+        engine.run_interval(10, 5)

@@ -1,0 +1,857 @@
+// api.hip -- the C ABI of include/voss/voss.h: context, planning of LDS
+// windows, kernel launches and the host finalisation that turns the device
+// accumulators into the reference's output arrays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mr_launch.h"
+#include "sieve_launch.h"
+#include "voss/voss.h"
+#include "wheel.h"
+
+using namespace voss;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(VOSS_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(call)                                         \
+    do {                                                     \
+        hipError_t _e = (call);                              \
+        if (_e != hipSuccess) return hip_fail(_e, #call);    \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(n, 64);
+        hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+constexpr uint32_t kW30Mask[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+
+uint64_t isqrt_u64(uint64_t n) {
+    uint64_t r = (uint64_t)std::sqrt((long double)n);
+    while (r > 0 && r * r > n) r--;
+    while ((r + 1) * (r + 1) <= n) r++;
+    return r;
+}
+
+// Primes 7 <= p <= limit (segmented odd-only sieve; limit < 2^32).
+std::vector<uint32_t> make_base_primes(uint64_t limit) {
+    std::vector<uint32_t> out;
+    if (limit < 7) return out;
+    const uint64_t root = isqrt_u64(limit);
+    std::vector<uint8_t> small(root + 1, 1);
+    std::vector<uint32_t> sp;
+    for (uint64_t i = 2; i <= root; i++) {
+        if (!small[i]) continue;
+        if (i > 2) sp.push_back((uint32_t)i);
+        for (uint64_t j = i * i; j <= root; j += i) small[j] = 0;
+    }
+    const uint64_t SEG = 1 << 20;  // odd numbers per segment
+    std::vector<uint8_t> seg(SEG);
+    for (uint64_t lo = 1; lo <= limit; lo += 2 * SEG) {
+        // seg[i] <-> lo + 2i
+        const uint64_t hi = std::min<uint64_t>(lo + 2 * SEG - 1, limit);
+        const uint64_t n = (hi - lo) / 2 + 1;
+        std::fill(seg.begin(), seg.begin() + n, 1);
+        for (uint32_t q : sp) {
+            const uint64_t q2 = (uint64_t)q * q;
+            if (q2 > hi) break;
+            uint64_t st = q2 >= lo ? q2 : ((lo + q - 1) / q) * q;
+            if ((st & 1) == 0) st += q;
+            for (uint64_t m = (st - lo) / 2; m < n; m += q) seg[m] = 0;
+        }
+        for (uint64_t i = 0; i < n; i++) {
+            const uint64_t v = lo + 2 * i;
+            if (seg[i] && v >= 7) out.push_back((uint32_t)v);
+        }
+    }
+    return out;
+}
+
+float merit_of(uint64_t p_before, uint32_t g) {
+    return (float)((double)g / std::log((double)p_before));
+}
+
+bool merit_ge10(uint64_t p_before, uint32_t g) {
+    return (double)g / std::log((double)p_before) >= 10.0;
+}
+
+}  // namespace
+
+struct voss_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    std::string name;
+    int num_cus = 0;
+    double total_gb = 0;
+    voss_tuning tune{131072, 1, 1024, 0};
+
+    std::vector<uint32_t> primes;
+    uint64_t primes_limit = 0;
+    DevBuf<uint32_t> d_primes;
+    DevBuf<uint64_t> d_magic;
+    size_t d_nprimes = 0;
+
+    DevBuf<uint32_t> d_pat;
+    uint32_t pat_off[kPresieveGroups] = {};
+    uint32_t pat_len[kPresieveGroups] = {};
+    uint32_t restore0 = 0;
+
+    std::vector<unsigned long long> lg_thresh;
+    DevBuf<unsigned long long> d_lg_thresh;
+
+    DevBuf<SegDesc> d_desc;
+    DevBuf<SegRec> d_rec;
+    DevBuf<JobAcc> d_acc;
+    DevBuf<LargeRaw> d_large;
+    DevBuf<unsigned long long> d_large_n;
+
+    DevBuf<uint64_t> d_mr_n;
+    DevBuf<uint8_t> d_mr_v;
+    DevBuf<uint32_t> d_surv;
+    DevBuf<uint32_t> d_nsurv;
+};
+
+namespace {
+
+int set_device(voss_ctx* c) {
+    HIPCHK(hipSetDevice(c->device));
+    return VOSS_OK;
+}
+
+// Base primes covering every window that ends at or below max_number.
+int ensure_primes(voss_ctx* c, uint64_t max_number) {
+    const uint64_t need = isqrt_u64(max_number) + 1;
+    if (need <= c->primes_limit && c->d_nprimes == c->primes.size()) return VOSS_OK;
+    if (need > 0xFFFFFFFFull) return fail(VOSS_EINVAL, "range beyond 2^64");
+    uint64_t lim = std::max<uint64_t>(need, std::max<uint64_t>(c->primes_limit * 2, 1 << 16));
+    lim = std::min<uint64_t>(lim, 0xFFFFFFFFull);
+    c->primes = make_base_primes(lim);
+    c->primes_limit = lim;
+    std::vector<uint64_t> magic(c->primes.size());
+    for (size_t i = 0; i < c->primes.size(); i++) magic[i] = ~0ull / c->primes[i];
+    const size_t n = std::max<size_t>(c->primes.size(), 1);
+    HIPCHK(c->d_primes.reserve(n));
+    HIPCHK(c->d_magic.reserve(n));
+    if (!c->primes.empty()) {
+        HIPCHK(hipMemcpy(c->d_primes.p, c->primes.data(), c->primes.size() * 4,
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_magic.p, magic.data(), magic.size() * 8,
+                         hipMemcpyHostToDevice));
+    }
+    c->d_nprimes = c->primes.size();
+    return VOSS_OK;
+}
+
+uint32_t primes_upto(const voss_ctx* c, uint64_t max_number) {
+    const uint64_t r = isqrt_u64(max_number);
+    return (uint32_t)(std::upper_bound(c->primes.begin(), c->primes.end(),
+                                       (uint32_t)std::min<uint64_t>(r, 0xFFFFFFFFull)) -
+                      c->primes.begin());
+}
+
+// Presieve tables: for each group of three primes, one word per residue
+// of the word index modulo the product (a period of the group's pattern).
+int build_patterns(voss_ctx* c) {
+    std::vector<uint32_t> all;
+    for (int g = 0; g < kPresieveGroups; g++) {
+        const uint32_t* q = (const uint32_t*)kPresieveGroupPrimes[g];
+        const uint32_t P = q[0] * q[1] * q[2];
+        c->pat_off[g] = (uint32_t)all.size();
+        c->pat_len[g] = P;
+        std::vector<uint8_t> bytes(4 * (size_t)P, 0xFF);
+        for (int k = 0; k < 3; k++) {
+            const uint32_t p = q[k];
+            // byte k' of the period: numbers 30k'+W30[o]; crossing every
+            // multiple of p (the prime itself is restored in window 0)
+            for (int o = 0; o < 8; o++) {
+                // first byte b with 30b + W30[o] = 0 mod p
+                uint32_t b0 = 0;
+                while ((30u * b0 + kW30Mask[o]) % p) b0++;
+                for (uint64_t b = b0; b < bytes.size(); b += p) bytes[b] &= (uint8_t)~(1u << o);
+            }
+        }
+        for (uint32_t w = 0; w < P; w++) {
+            uint32_t v;
+            std::memcpy(&v, &bytes[4 * (size_t)w], 4);
+            all.push_back(v);
+        }
+    }
+    // word 0 (numbers 1..119): put back the presieved primes themselves
+    uint32_t restore = 0;
+    for (int g = 0; g < kPresieveGroups; g++)
+        for (int k = 0; k < 3; k++) {
+            const uint32_t p = (uint32_t)kPresieveGroupPrimes[g][k];
+            int o = -1;
+            for (int i = 0; i < 8; i++) if ((uint32_t)kW30[i] == p % 30) o = i;
+            restore |= 1u << ((p / 30) * 8 + o);
+        }
+    c->restore0 = restore;
+    HIPCHK(c->d_pat.reserve(all.size()));
+    HIPCHK(hipMemcpy(c->d_pat.p, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+    return VOSS_OK;
+}
+
+// Largest p_before for which float merit would reach 10 at gap g:
+// (double)g / log((double)p) >= 10.0 is monotone in p, so binary search.
+int build_lg_thresh(voss_ctx* c) {
+    c->lg_thresh.assign(kHistBins, 0);
+    for (int h = 0; h < kHistBins; h++) {
+        const uint32_t g = 2u * (uint32_t)h;
+        if (g <= 100) { c->lg_thresh[h] = 0; continue; }
+        if (g >= (uint32_t)kLargeAlways) { c->lg_thresh[h] = ~0ull; continue; }
+        if (merit_ge10(~0ull, g)) { c->lg_thresh[h] = ~0ull; continue; }
+        uint64_t lo = 2, hi = ~0ull;  // pred(lo) true, pred(hi) false
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (merit_ge10(mid, g)) lo = mid; else hi = mid;
+        }
+        c->lg_thresh[h] = lo;
+    }
+    HIPCHK(c->d_lg_thresh.reserve(kHistBins));
+    HIPCHK(hipMemcpy(c->d_lg_thresh.p, c->lg_thresh.data(), kHistBins * 8,
+                     hipMemcpyHostToDevice));
+    return VOSS_OK;
+}
+
+// Host-side accumulation of one job (the device accumulators plus the pairs
+// joined on the host: the head of the range and launch boundaries).
+struct JobHost {
+    std::vector<int64_t> gaps;     // [VOSS_MAX_GAP]
+    std::vector<int64_t> modq_gap; // [VOSS_MODQ_ROWS * VOSS_MODQ_MAX_GAP]
+    uint64_t count = 0, c1 = 0, mod30[8] = {};
+    uint64_t first = 0, last = 0;
+    uint64_t large_count = 0;
+    std::vector<voss_large_gap> large;
+    JobHost() : gaps(VOSS_MAX_GAP, 0), modq_gap(VOSS_MODQ_ROWS * VOSS_MODQ_MAX_GAP, 0) {}
+};
+
+void host_pair(JobHost& J, uint64_t p, uint64_t q, bool full) {
+    const uint64_t g = q - p;
+    if (g == 0) return;
+    if (g < VOSS_MAX_GAP) J.gaps[g]++;
+    if (!full || p < 7) return;
+    if (g < VOSS_MODQ_MAX_GAP) {
+        static const int q6[8] = {0, 0, 1, 0, 1, 0, 1, 1};
+        int o = -1;
+        for (int i = 0; i < 8; i++) if ((uint64_t)kW30[i] == p % 30) o = i;
+        J.modq_gap[q6[o] * VOSS_MODQ_MAX_GAP + g]++;
+        J.modq_gap[(2 + o) * VOSS_MODQ_MAX_GAP + g]++;
+    }
+    if (g > 100 && g < VOSS_MAX_GAP && (g >= (uint64_t)kLargeAlways || merit_ge10(p, (uint32_t)g))) {
+        J.large_count++;
+        J.large.push_back({q, (int32_t)g, merit_of(p, (uint32_t)g)});
+    }
+}
+
+void add_acc(JobHost& J, const JobAcc& A) {
+    for (int h = 1; h < kHistBins; h++) J.gaps[2 * h] += (int64_t)A.hist[h];
+    for (int r = 0; r < kModqRows; r++)
+        for (int h = 1; h < kModqBins; h++)
+            J.modq_gap[r * VOSS_MODQ_MAX_GAP + 2 * h] += (int64_t)A.modq[r * kModqBins + h];
+    J.count += A.count;
+    J.c1 += A.c1;
+    for (int o = 0; o < 8; o++) J.mod30[o] += A.mod30[o];
+    if (A.count) {
+        if (J.first == 0 || A.first < J.first) J.first = A.first;
+        if (A.last > J.last) J.last = A.last;
+    }
+    J.large_count += A.large_count;
+}
+
+bool lg_less(const voss_large_gap& a, const voss_large_gap& b) {
+    if (a.merit != b.merit) return a.merit > b.merit;
+    return a.position < b.position;
+}
+
+void write_result(JobHost& J, voss_result* out) {
+    std::memcpy(out->gaps, J.gaps.data(), sizeof(out->gaps));
+    std::memcpy(out->modq_gap, J.modq_gap.data(), sizeof(out->modq_gap));
+    out->large_count = J.large_count;
+    std::sort(J.large.begin(), J.large.end(), lg_less);
+    if (out->large && out->large_capacity) {
+        const size_t n = std::min<size_t>(J.large.size(), out->large_capacity);
+        std::memcpy(out->large, J.large.data(), n * sizeof(voss_large_gap));
+    }
+}
+
+// Wheel-prime contribution to the residue counters (slot layout of the
+// reference's modq_d, src/voss_v7_live.cu:195-232).
+void wheel_modq(const JobHost& J, uint64_t* modq) {
+    for (int o = 0; o < 8; o++) {
+        modq[kW30[o] % 6] += J.mod30[o];
+        modq[6 + kW30[o]] += J.mod30[o];
+    }
+}
+
+struct Plan {
+    std::vector<SegDesc> desc;
+    uint32_t njobs = 0;
+};
+
+// Windows of the grid w0 = k*W covering numbers [a, b] (a >= 7).
+void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job,
+                uint32_t lseg, uint32_t W, uint64_t grid_origin) {
+    if (a < 7) a = 7;
+    if (a > b) return;
+    const uint64_t ka = (a / 30 - grid_origin) / W, kb = (b / 30 - grid_origin) / W;
+    for (uint64_t k = ka; k <= kb; k++) {
+        const uint64_t w0 = grid_origin + k * W;
+        const uint64_t lo = std::max<uint64_t>(a, 30 * w0);
+        const uint64_t hi = std::min<uint64_t>(b, 30 * (w0 + W) - 1);
+        if (lo > hi) continue;
+        uint64_t need = hi / 30 - w0 + 1;
+        need = (need + 4095) / 4096 * 4096;
+        SegDesc d;
+        d.w0 = w0;
+        d.n_lo = lo;
+        d.n_hi = hi;
+        d.len = (uint32_t)std::min<uint64_t>(need, W);
+        d.nprimes = primes_upto(c, 30 * (w0 + d.len) - 1);
+        d.job = job;
+        d.lseg = lseg;
+        P.desc.push_back(d);
+    }
+}
+
+SieveArgs make_args(voss_ctx* c, uint32_t ndesc, unsigned long long large_cap) {
+    SieveArgs a{};
+    a.desc = c->d_desc.p;
+    a.ndesc = ndesc;
+    a.tier_a_begin = kPresievePrimes;
+    a.tier_a_end = (uint32_t)(std::lower_bound(c->primes.begin(), c->primes.end(),
+                                               c->tune.tier_a_limit) - c->primes.begin());
+    if (a.tier_a_end < a.tier_a_begin) a.tier_a_end = a.tier_a_begin;
+    a.restore0 = c->restore0;
+    a.primes = c->d_primes.p;
+    a.magic = c->d_magic.p;
+    const int block = sieve_block_threads((int)c->tune.window_bytes);
+    for (int g = 0; g < kPresieveGroups; g++) {
+        a.pat[g] = c->d_pat.p + c->pat_off[g];
+        a.pat_len[g] = c->pat_len[g];
+        a.pat_step[g] = (uint32_t)block % c->pat_len[g];
+    }
+    a.lg_thresh = c->d_lg_thresh.p;
+    a.rec = c->d_rec.p;
+    a.acc = c->d_acc.p;
+    a.large = c->d_large.p;
+    a.large_cap = large_cap;
+    a.large_n = c->d_large_n.p;
+    return a;
+}
+
+constexpr size_t kMaxDescPerLaunch = 1u << 20;
+
+// Runs the plan: sieve + stitch per launch of at most kMaxDescPerLaunch
+// windows, joining launch boundaries on the host.  jobs[j] receives the
+// accumulated result of job j.  large_cap_total bounds the device list.
+struct ExecTimes {
+    double device_ms = 0;  // sieve + stitch
+    double sieve_ms = 0;   // sieve kernel alone
+};
+
+int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
+            uint64_t large_cap_total, ExecTimes* times) {
+    jobs.assign(P.njobs, JobHost());
+    if (P.njobs == 0) return VOSS_OK;
+    HIPCHK(c->d_acc.reserve(P.njobs));
+    HIPCHK(c->d_large_n.reserve(1));
+    const uint64_t lcap = std::max<uint64_t>(large_cap_total, 1);
+    HIPCHK(c->d_large.reserve(lcap));
+    const uint32_t block_per_cu = std::max<uint32_t>(c->tune.wg_per_cu, 1);
+    const uint32_t grid = (uint32_t)c->num_cus * block_per_cu;
+
+    // carry across launch boundaries: last non-empty window per job
+    std::vector<uint64_t> carry_last(P.njobs, 0);
+    std::vector<uint32_t> carry_lseg(P.njobs, 0);
+    std::vector<uint8_t> carry_has(P.njobs, 0);
+    std::vector<SegRec> rec;
+    std::vector<JobAcc> acc(P.njobs);
+    std::vector<LargeRaw> raw;
+    float total_ms = 0, sieve_ms = 0;
+
+    for (size_t off = 0; off < P.desc.size(); off += kMaxDescPerLaunch) {
+        const uint32_t nd = (uint32_t)std::min(kMaxDescPerLaunch, P.desc.size() - off);
+        HIPCHK(c->d_desc.reserve(nd));
+        HIPCHK(c->d_rec.reserve(nd));
+        HIPCHK(hipMemcpyAsync(c->d_desc.p, P.desc.data() + off, nd * sizeof(SegDesc),
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_init_acc(c->d_acc.p, P.njobs, c->stream));
+        HIPCHK(hipMemsetAsync(c->d_large_n.p, 0, sizeof(unsigned long long), c->stream));
+        SieveArgs a = make_args(c, nd, lcap);
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, c->stream));
+        HIPCHK(hipEventRecord(c->ev2, c->stream));
+        HIPCHK(launch_stitch(a, c->stream));
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipMemcpyAsync(acc.data(), c->d_acc.p, P.njobs * sizeof(JobAcc),
+                              hipMemcpyDeviceToHost, c->stream));
+        unsigned long long nlarge = 0;
+        HIPCHK(hipMemcpyAsync(&nlarge, c->d_large_n.p, 8, hipMemcpyDeviceToHost, c->stream));
+        rec.resize(nd);
+        HIPCHK(hipMemcpyAsync(rec.data(), c->d_rec.p, nd * sizeof(SegRec),
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0, sms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        HIPCHK(hipEventElapsedTime(&sms, c->ev0, c->ev2));
+        total_ms += ms;
+        sieve_ms += sms;
+        const uint64_t nkeep = std::min<uint64_t>(nlarge, lcap);
+        raw.resize(nkeep);
+        if (nkeep)
+            HIPCHK(hipMemcpy(raw.data(), c->d_large.p, nkeep * sizeof(LargeRaw),
+                             hipMemcpyDeviceToHost));
+        for (uint32_t j = 0; j < P.njobs; j++) {
+            if (acc[j].error) return fail(VOSS_ERANGE, "prime gap >= 2048 (beyond the device histogram)");
+            add_acc(jobs[j], acc[j]);
+        }
+        for (const LargeRaw& r : raw)
+            jobs[r.job].large.push_back({r.p_before + r.gap, (int32_t)r.gap,
+                                         merit_of(r.p_before, r.gap)});
+        // launch boundary: pair the previous launch's last prime of a job
+        // with this launch's first non-empty window of the same job
+        std::vector<uint8_t> seen(P.njobs, 0);
+        for (uint32_t i = 0; i < nd; i++) {
+            const SegDesc& d = P.desc[off + i];
+            if (rec[i].count == 0) continue;
+            if (!seen[d.job]) {
+                seen[d.job] = 1;
+                if (carry_has[d.job])
+                    host_pair(jobs[d.job], carry_last[d.job], rec[i].first,
+                              carry_lseg[d.job] == d.lseg);
+            }
+            carry_has[d.job] = 1;
+            carry_last[d.job] = rec[i].last;
+            carry_lseg[d.job] = d.lseg;
+        }
+    }
+    if (times) {
+        times->device_ms = total_ms;
+        times->sieve_ms = sieve_ms;
+    }
+    return VOSS_OK;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+int check_result(voss_result* out) {
+    if (!out) return fail(VOSS_EINVAL, "result pointer is NULL");
+    if (out->large_capacity && !out->large) return fail(VOSS_EINVAL, "large buffer is NULL");
+    return VOSS_OK;
+}
+
+void clear_result(voss_result* out) {
+    voss_large_gap* lg = out->large;
+    const uint64_t cap = out->large_capacity;
+    std::memset(out, 0, sizeof(*out));
+    out->large = lg;
+    out->large_capacity = cap;
+}
+
+// Interval bookkeeping shared by the single and batched forms: primes
+// 2, 3, 5 of [lo, hi) precede the wheel primes.
+void finish_interval(JobHost& J, uint64_t lo, uint64_t hi, voss_result* out) {
+    clear_result(out);
+    uint64_t total = 0, first = 0, last = 0;
+    uint64_t c1 = 0, c3 = 0;
+    uint64_t modq[VOSS_Q_SLOTS] = {};
+    static const uint64_t small[3] = {2, 3, 5};
+    for (uint64_t p : small) {
+        if (p < lo || p >= hi) continue;
+        if ((p & 3) == 1) c1++;
+        if ((p & 3) == 3) c3++;
+        modq[p % 6]++;
+        modq[6 + p % 30]++;
+        if (total) host_pair(J, last, p, true);
+        else first = p;
+        last = p;
+        total++;
+    }
+    if (J.count) {
+        if (total) host_pair(J, last, J.first, true);
+        else first = J.first;
+        last = J.last;
+    }
+    total += J.count;
+    wheel_modq(J, modq);
+    out->total_primes = total;
+    out->first_prime = first;
+    out->last_prime = last;
+    out->c1 = c1 + J.c1;
+    out->c3 = c3 + (J.count - J.c1);
+    std::memcpy(out->modq, modq, sizeof(modq));
+    write_result(J, out);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* voss_last_error(void) { return g_err.c_str(); }
+
+int voss_create(int device, voss_ctx** out) {
+    if (!out) return fail(VOSS_EINVAL, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(VOSS_EINVAL, "no such device");
+    voss_ctx* c = new voss_ctx();
+    c->device = device;
+    int rc = set_device(c);
+    hipDeviceProp_t prop;
+    if (rc == VOSS_OK) {
+        hipError_t e = hipGetDeviceProperties(&prop, device);
+        if (e != hipSuccess) rc = hip_fail(e, "hipGetDeviceProperties");
+    }
+    if (rc == VOSS_OK) {
+        c->name = prop.name;
+        c->num_cus = prop.multiProcessorCount;
+        c->total_gb = prop.totalGlobalMem / 1e9;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+        if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+        if (e == hipSuccess) e = hipEventCreate(&c->ev2);
+        if (e == hipSuccess) e = upload_sieve_constants(device);
+        if (e != hipSuccess) rc = hip_fail(e, "context setup");
+    }
+    if (rc == VOSS_OK) rc = build_patterns(c);
+    if (rc == VOSS_OK) rc = build_lg_thresh(c);
+    if (rc == VOSS_OK) rc = ensure_primes(c, 1ull << 32);
+    if (rc != VOSS_OK) {
+        voss_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return VOSS_OK;
+}
+
+void voss_destroy(voss_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->d_primes.release();
+    c->d_magic.release();
+    c->d_pat.release();
+    c->d_lg_thresh.release();
+    c->d_desc.release();
+    c->d_rec.release();
+    c->d_acc.release();
+    c->d_large.release();
+    c->d_large_n.release();
+    c->d_mr_n.release();
+    c->d_mr_v.release();
+    c->d_surv.release();
+    c->d_nsurv.release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int voss_device_name(voss_ctx* c, char* buf, size_t cap, int* num_cus, double* total_gb) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    if (buf && cap) {
+        std::snprintf(buf, cap, "%s", c->name.c_str());
+    }
+    if (num_cus) *num_cus = c->num_cus;
+    if (total_gb) *total_gb = c->total_gb;
+    return VOSS_OK;
+}
+
+int voss_set_tuning(voss_ctx* c, const voss_tuning* t) {
+    if (!c || !t) return fail(VOSS_EINVAL, "NULL argument");
+    voss_tuning n = c->tune;
+    if (t->window_bytes) {
+        if (sieve_block_threads((int)t->window_bytes) == 0)
+            return fail(VOSS_EINVAL, "window_bytes must be 32768, 65536 or 131072");
+        n.window_bytes = t->window_bytes;
+    }
+    if (t->wg_per_cu) n.wg_per_cu = t->wg_per_cu;
+    if (t->tier_a_limit) n.tier_a_limit = t->tier_a_limit;
+    c->tune = n;
+    return VOSS_OK;
+}
+
+int voss_get_tuning(voss_ctx* c, voss_tuning* t) {
+    if (!c || !t) return fail(VOSS_EINVAL, "NULL argument");
+    *t = c->tune;
+    return VOSS_OK;
+}
+
+int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    int rc = check_result(out);
+    if (rc) return rc;
+    if (N < 5) return fail(VOSS_EINVAL, "N must be >= 5");
+    if (seg_num == 0 || seg_num % 30 != 0) return fail(VOSS_EINVAL, "SEG_NUM must be a positive multiple of 30");
+    if (N > (1ull << 62)) return fail(VOSS_EINVAL, "N beyond 2^62");
+    if ((rc = set_device(c))) return rc;
+    const uint32_t W = c->tune.window_bytes;
+    if ((rc = ensure_primes(c, (N / 30 / W + 1) * W * 30))) return rc;
+
+    const double t0 = now_s();
+    Plan P;
+    P.njobs = 1;
+    const uint64_t num_seg = (N + seg_num - 1) / seg_num;  // :342
+    for (uint64_t s = 0; s < num_seg; s++) {
+        const uint64_t a = s * seg_num + 1;
+        const uint64_t b = std::min<uint64_t>((s + 1) * seg_num, N);
+        plan_range(c, P, a, b, 0, (uint32_t)s, W, 0);
+    }
+    std::vector<JobHost> jobs;
+    ExecTimes tm;
+    const uint64_t cap = out->large ? out->large_capacity : 0;
+    if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm))) return rc;
+    JobHost& J = jobs[0];
+
+    clear_result(out);
+    // (5, first wheel prime) is the reference's host boundary gap of
+    // segment 0 (:559-562): main histogram only
+    if (J.count) host_pair(J, 5, J.first, false);
+    J.gaps[1] += 1;  // (2,3)  :634
+    J.gaps[2] += 1;  // (3,5)  :635
+    out->total_primes = 3 + J.count;
+    out->first_prime = 2;
+    out->last_prime = J.count ? J.last : 5;
+    out->c1 = J.c1;
+    out->c3 = J.count - J.c1;
+    wheel_modq(J, out->modq);
+    write_result(J, out);
+    out->seconds = now_s() - t0;
+    out->device_ms = tm.device_ms;
+    out->sieve_ms = tm.sieve_ms;
+    out->windows = P.desc.size();
+    return VOSS_OK;
+}
+
+int voss_run_interval(voss_ctx* c, uint64_t lo, uint64_t hi, voss_result* out) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    int rc = check_result(out);
+    if (rc) return rc;
+    if (hi < lo) return fail(VOSS_EINVAL, "hi < lo");
+    if (hi > (1ull << 62)) return fail(VOSS_EINVAL, "hi beyond 2^62");
+    if ((rc = set_device(c))) return rc;
+    const uint32_t W = c->tune.window_bytes;
+    if ((rc = ensure_primes(c, hi / 30 * 30 + 30ull * W + 30))) return rc;
+    const double t0 = now_s();
+    Plan P;
+    P.njobs = 1;
+    if (hi > 7) {
+        const uint64_t a = std::max<uint64_t>(lo, 7), b = hi - 1;
+        plan_range(c, P, a, b, 0, 0, W, (a / 30) & ~4095ull);
+    }
+    std::vector<JobHost> jobs;
+    ExecTimes tm;
+    const uint64_t cap = out->large ? out->large_capacity : 0;
+    if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm))) return rc;
+    finish_interval(jobs[0], lo, hi, out);
+    out->seconds = now_s() - t0;
+    out->device_ms = tm.device_ms;
+    out->sieve_ms = tm.sieve_ms;
+    out->windows = P.desc.size();
+    return VOSS_OK;
+}
+
+int voss_run_batch(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
+                   voss_result* out) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    if (n && (!lo || !hi || !out)) return fail(VOSS_EINVAL, "NULL argument");
+    if (n > 0xFFFFFFFFull) return fail(VOSS_EINVAL, "too many intervals");
+    int rc;
+    uint64_t maxhi = 0, cap = 0;
+    for (size_t i = 0; i < n; i++) {
+        if ((rc = check_result(&out[i]))) return rc;
+        if (hi[i] < lo[i]) return fail(VOSS_EINVAL, "hi < lo");
+        if (hi[i] > (1ull << 62)) return fail(VOSS_EINVAL, "hi beyond 2^62");
+        maxhi = std::max(maxhi, hi[i]);
+        if (out[i].large) cap += out[i].large_capacity;
+    }
+    if ((rc = set_device(c))) return rc;
+    const uint32_t W = c->tune.window_bytes;
+    if ((rc = ensure_primes(c, maxhi / 30 * 30 + 30ull * W + 30))) return rc;
+    const double t0 = now_s();
+    Plan P;
+    P.njobs = (uint32_t)n;
+    for (size_t i = 0; i < n; i++) {
+        if (hi[i] <= 7) continue;
+        const uint64_t a = std::max<uint64_t>(lo[i], 7), b = hi[i] - 1;
+        plan_range(c, P, a, b, (uint32_t)i, 0, W, (a / 30) & ~4095ull);
+    }
+    std::vector<JobHost> jobs;
+    ExecTimes tm;
+    if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm))) return rc;
+    const double dt = now_s() - t0;
+    for (size_t i = 0; i < n; i++) {
+        finish_interval(jobs[i], lo[i], hi[i], &out[i]);
+        out[i].seconds = dt;
+        out[i].device_ms = tm.device_ms;
+        out[i].sieve_ms = tm.sieve_ms;
+        out[i].windows = P.desc.size();
+    }
+    return VOSS_OK;
+}
+
+int voss_run_batch_plain(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
+                         voss_result* out) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    if (n && (!lo || !hi || !out)) return fail(VOSS_EINVAL, "NULL argument");
+    for (size_t i = 0; i < n; i++) {
+        int rc = voss_run_interval(c, lo[i], hi[i], &out[i]);
+        if (rc) return rc;
+    }
+    return VOSS_OK;
+}
+
+int voss_sample_plain(voss_ctx*, uint64_t, uint64_t, int32_t*, uint64_t*, uint64_t*) {
+    return fail(VOSS_EINVAL, "gap_sequence sampling is not built yet");
+}
+
+int voss_mr_batch_device(voss_ctx* c, const uint64_t* d_n, size_t count, uint8_t* d_verdict,
+                         double* kernel_ms) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    if (count && (!d_n || !d_verdict)) return fail(VOSS_EINVAL, "NULL buffer");
+    if (count >= 0xFFFFFFFFull) return fail(VOSS_EINVAL, "count must be below 2^32");
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    if (count == 0) {
+        if (kernel_ms) *kernel_ms = 0;
+        return VOSS_OK;
+    }
+    HIPCHK(c->d_surv.reserve(count));
+    HIPCHK(c->d_nsurv.reserve(1));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(launch_mr(d_n, count, d_verdict, c->d_surv.p, c->d_nsurv.p, c->num_cus, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    return VOSS_OK;
+}
+
+int voss_mr_batch(voss_ctx* c, const uint64_t* n, size_t count, uint8_t* verdict) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    if (count && (!n || !verdict)) return fail(VOSS_EINVAL, "NULL buffer");
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    if (count == 0) return VOSS_OK;
+    HIPCHK(c->d_mr_n.reserve(count));
+    HIPCHK(c->d_mr_v.reserve(count));
+    HIPCHK(hipMemcpyAsync(c->d_mr_n.p, n, count * 8, hipMemcpyHostToDevice, c->stream));
+    if ((rc = voss_mr_batch_device(c, c->d_mr_n.p, count, c->d_mr_v.p, nullptr))) return rc;
+    HIPCHK(hipMemcpy(verdict, c->d_mr_v.p, count, hipMemcpyDeviceToHost));
+    return VOSS_OK;
+}
+
+int voss_verify_gaps(voss_ctx* c, const uint64_t* p_after, const int32_t* gap, size_t count,
+                     uint8_t* ok) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    if (count && (!p_after || !gap || !ok)) return fail(VOSS_EINVAL, "NULL buffer");
+    if (count == 0) return VOSS_OK;
+    // both ends: GPU Miller-Rabin (verify_gap :61-62)
+    std::vector<uint64_t> ends(2 * count);
+    for (size_t i = 0; i < count; i++) {
+        ends[2 * i] = p_after[i] - (uint64_t)(int64_t)gap[i];
+        ends[2 * i + 1] = p_after[i];
+    }
+    std::vector<uint8_t> v(2 * count);
+    int rc = voss_mr_batch(c, ends.data(), ends.size(), v.data());
+    if (rc) return rc;
+    // interior: no prime strictly between the ends (:63-92), GPU interval
+    // sieve over (p_before, p_after) for the gaps whose ends are prime
+    std::vector<uint64_t> lo, hi;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < count; i++) {
+        ok[i] = (uint8_t)(v[2 * i] && v[2 * i + 1]);
+        if (ok[i] && gap[i] >= 2) {
+            lo.push_back(ends[2 * i] + 1);
+            hi.push_back(p_after[i]);
+            idx.push_back(i);
+        }
+    }
+    const size_t kChunk = 4096;
+    std::vector<voss_result> res;
+    for (size_t off = 0; off < lo.size(); off += kChunk) {
+        const size_t m = std::min(kChunk, lo.size() - off);
+        res.assign(m, voss_result());
+        for (auto& r : res) { r.large = nullptr; r.large_capacity = 0; }
+        if ((rc = voss_run_batch(c, lo.data() + off, hi.data() + off, m, res.data()))) return rc;
+        for (size_t k = 0; k < m; k++)
+            if (res[k].total_primes != 0) ok[idx[off + k]] = 0;
+    }
+    return VOSS_OK;
+}
+
+int voss_device_alloc(voss_ctx* c, size_t bytes, void** dptr) {
+    if (!c || !dptr) return fail(VOSS_EINVAL, "NULL argument");
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(hipMalloc(dptr, bytes));
+    return VOSS_OK;
+}
+
+int voss_device_free(voss_ctx* c, void* dptr) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(hipFree(dptr));
+    return VOSS_OK;
+}
+
+int voss_memcpy_h2d(voss_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return VOSS_OK;
+}
+
+int voss_memcpy_d2h(voss_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return VOSS_OK;
+}
+
+}  // extern "C"

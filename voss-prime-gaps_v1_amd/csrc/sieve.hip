@@ -1,0 +1,445 @@
+// sieve.hip -- the fused wheel-30 segment sieve for gfx950.
+//
+// Replaces, in one persistent kernel, the reference's per-segment chain
+// (src/voss_v7_live.cu): one sieve_w30_seg_kernel launch per base prime
+// over a 1 GB HBM bitmap (:120-143, :480-502), extract_w30_seg_kernel
+// (:145-179), thrust::sort (:522-524), mod4_count_kernel /
+// modq_count_kernel (:181-232) and gaps_w30_seg_kernel (:255-312).
+//
+// Design (one workgroup = one LDS window at a time, grid-stride over the
+// window descriptors):
+//   phase 0  the window's bitmap is initialised in LDS from six presieve
+//            pattern tables (primes 7..73), one AND per table per word;
+//   phase 1  every other base prime p with p*p <= window end crosses off
+//            its 8 residue classes with ds_and_b32.  Primes below
+//            tier_a_end are marked by a whole wave (8 lanes per class,
+//            stride 8p); larger ones by one lane each (all 8 classes).
+//            Waves pull work from an LDS counter, smallest primes first,
+//            so the phase is balanced without a static split;
+//   phase 2  each thread walks its contiguous run of words in increasing
+//            order: popcounts give the per-residue and mod-4 counts, the
+//            set bits give consecutive primes, whose gaps go to LDS
+//            histograms; gaps that cross thread runs are joined after a
+//            barrier; gaps that cross windows are joined by stitch_kernel.
+// The bitmap never leaves LDS.  HBM traffic per window is the descriptor,
+// one 24-byte record, and the (rare) large-gap records; histograms are
+// merged into HBM once per workgroup per job.
+#include <hip/hip_runtime.h>
+
+#include "sieve_launch.h"
+#include "wheel.h"
+
+namespace voss {
+
+__constant__ uint8_t c_bitidx[8][8];  // [idx(p%30)][j] -> bit of p*W30[j]
+__constant__ int8_t c_residx[30];     // residue -> wheel bit, -1 if none
+__constant__ int8_t c_firstbit[30];   // smallest o with W30[o] >= r
+__constant__ int8_t c_lastbit[30];    // largest o with W30[o] <= r, or -1
+__constant__ uint32_t c_w30[8];
+__constant__ uint32_t c_q6row[8];     // W30_TO_Q6_IDX (src/voss_v7_live.cu:30)
+
+// x mod p for any 64-bit x: m = floor(2^64 / p) (Barrett, one correction).
+__device__ __forceinline__ uint32_t mod_u64(uint64_t x, uint32_t p,
+                                            uint64_t m) {
+    uint64_t q = __umul64hi(x, m);
+    uint64_t r = x - q * (uint64_t)p;
+    return (uint32_t)(r >= p ? r - p : r);
+}
+
+__device__ __forceinline__ void clear_bit(uint32_t* bm, uint32_t b,
+                                          uint32_t bit) {
+    atomicAnd(&bm[b >> 2], ~(1u << (((b & 3u) << 3) | bit)));
+}
+
+// Position of the first multiple of p in residue class j inside the window
+// starting at byte w0 (relative byte index, may be >= len).  The multiple
+// p*1 (the prime itself) is skipped; smaller composite multiples p*m with
+// 7 <= m < p may be crossed, which changes nothing.
+__device__ __forceinline__ uint32_t first_pos(uint32_t p, uint32_t r,
+                                              int j, uint64_t w0) {
+    uint32_t off = (uint32_t)(((uint64_t)p * c_w30[j]) / 30u);
+    uint32_t pos = off >= r ? off - r : off + p - r;
+    if (j == 0 && w0 <= (uint64_t)off) pos += p;
+    return pos;
+}
+
+struct PairCtx {
+    uint32_t* hist;
+    uint32_t* mq;
+    const SieveArgs* a;
+    unsigned long long base_num;
+    uint32_t job;
+};
+
+// One consecutive pair inside a window: offsets from base_num.
+__device__ __forceinline__ void pair_in_window(const PairCtx& c,
+                                               uint32_t pb, uint32_t pa,
+                                               uint32_t obit) {
+    uint32_t g = pa - pb;
+    if (g < 2u * kHistBins) {
+        atomicAdd(&c.hist[g >> 1], 1u);
+    } else {
+        atomicOr(&c.a->acc[c.job].error, 1ull);
+        return;
+    }
+    if (g < 100u) {
+        atomicAdd(&c.mq[c_q6row[obit] * kModqBins + (g >> 1)], 1u);
+        atomicAdd(&c.mq[(2u + obit) * kModqBins + (g >> 1)], 1u);
+    } else if (g > 100u) {
+        unsigned long long p_before = c.base_num + pb;
+        if (g >= (uint32_t)kLargeAlways || p_before <= c.a->lg_thresh[g >> 1]) {
+            unsigned long long slot = atomicAdd(c.a->large_n, 1ull);
+            atomicAdd(&c.a->acc[c.job].large_count, 1ull);
+            if (slot < c.a->large_cap) {
+                LargeRaw r;
+                r.p_before = p_before;
+                r.gap = g;
+                r.job = c.job;
+                c.a->large[slot] = r;
+            }
+        }
+    }
+}
+
+template <int WIN, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
+    constexpr int W4 = WIN / 4;
+    __shared__ uint32_t bm[W4];
+    __shared__ uint32_t hist[kHistBins];
+    __shared__ uint32_t mq[kModqRows * kModqBins];
+    __shared__ uint32_t tfirst[BLOCK];
+    __shared__ uint32_t tlast[BLOCK];
+    __shared__ unsigned long long s_cnt[10];
+    __shared__ uint32_t s_task;
+    __shared__ uint32_t s_patbase[kPresieveGroups];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+
+    for (int i = tid; i < kHistBins; i += BLOCK) hist[i] = 0;
+    for (int i = tid; i < kModqRows * kModqBins; i += BLOCK) mq[i] = 0;
+    uint32_t cur_job = blockIdx.x < a.ndesc ? a.desc[blockIdx.x].job : 0;
+
+    auto flush = [&](uint32_t job) {
+        __syncthreads();
+        JobAcc* acc = &a.acc[job];
+        for (int i = tid; i < kHistBins; i += BLOCK) {
+            uint32_t v = hist[i];
+            if (v) {
+                atomicAdd(&acc->hist[i], (unsigned long long)v);
+                hist[i] = 0;
+            }
+        }
+        for (int i = tid; i < kModqRows * kModqBins; i += BLOCK) {
+            uint32_t v = mq[i];
+            if (v) {
+                atomicAdd(&acc->modq[i], (unsigned long long)v);
+                mq[i] = 0;
+            }
+        }
+        __syncthreads();
+    };
+
+    for (uint32_t d = blockIdx.x; d < a.ndesc; d += gridDim.x) {
+        const SegDesc D = a.desc[d];
+        if (D.job != cur_job) {
+            flush(cur_job);
+            cur_job = D.job;
+        }
+        const uint32_t L4 = D.len >> 2;
+        const uint64_t dw0 = D.w0 >> 2;
+        if (tid < kPresieveGroups)
+            s_patbase[tid] = (uint32_t)(dw0 % a.pat_len[tid]);
+        if (tid == 0) s_task = 0;
+        if (tid < 10) s_cnt[tid] = 0;
+        __syncthreads();
+
+        // ---- phase 0: presieve patterns (primes 7..73)
+        {
+            uint32_t idx[kPresieveGroups];
+#pragma unroll
+            for (int g = 0; g < kPresieveGroups; g++) {
+                uint32_t P = a.pat_len[g];
+                uint32_t v = s_patbase[g] + (uint32_t)tid;
+                while (v >= P) v -= P;
+                idx[g] = v;
+            }
+            for (uint32_t i = tid; i < L4; i += BLOCK) {
+                uint32_t w = 0xFFFFFFFFu;
+#pragma unroll
+                for (int g = 0; g < kPresieveGroups; g++) {
+                    w &= a.pat[g][idx[g]];
+                    uint32_t v = idx[g] + a.pat_step[g];
+                    idx[g] = v >= a.pat_len[g] ? v - a.pat_len[g] : v;
+                }
+                if (i == 0 && D.w0 == 0) w = (w | a.restore0) & ~1u;
+                bm[i] = w;
+            }
+        }
+        __syncthreads();
+
+        // ---- phase 1: base primes 79 <= p, p*p <= window end
+        {
+            const uint32_t np = D.nprimes;
+            const uint32_t a_end = np < a.tier_a_end ? np : a.tier_a_end;
+            const uint32_t nA = a_end > a.tier_a_begin ? a_end - a.tier_a_begin : 0;
+            const uint32_t nB = np > a.tier_a_end ? (np - a.tier_a_end + 63) / 64 : 0;
+            const uint32_t ntask = nA + nB;
+            const uint32_t len = D.len;
+            while (true) {
+                uint32_t t = 0;
+                if (lane == 0) t = atomicAdd(&s_task, 1u);
+                t = __shfl(t, 0);
+                if (t >= ntask) break;
+                if (t < nA) {
+                    const uint32_t pi = a.tier_a_begin + t;
+                    const uint32_t p = a.primes[pi];
+                    const uint32_t r = mod_u64(D.w0, p, a.magic[pi]);
+                    const int j = lane >> 3;
+                    const uint32_t s = lane & 7;
+                    const uint32_t bit = c_bitidx[c_residx[p % 30u]][j];
+                    uint32_t b = first_pos(p, r, j, D.w0) + s * p;
+                    const uint32_t stride = 8u * p;
+                    for (; b < len; b += stride) clear_bit(bm, b, bit);
+                } else {
+                    const uint32_t pi = a.tier_a_end + (t - nA) * 64 + lane;
+                    if (pi < np) {
+                        const uint32_t p = a.primes[pi];
+                        const uint32_t r = mod_u64(D.w0, p, a.magic[pi]);
+                        const int pr = c_residx[p % 30u];
+#pragma unroll 1
+                        for (int j = 0; j < 8; j++) {
+                            const uint32_t bit = c_bitidx[pr][j];
+                            for (uint32_t b = first_pos(p, r, j, D.w0); b < len; b += p)
+                                clear_bit(bm, b, bit);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- phase 2: mask to [n_lo, n_hi], count, walk primes in order
+        const unsigned long long base_num = 30ull * D.w0;
+        int bitlo = 0, bithi = (int)(8u * D.len) - 1;
+        if (D.n_lo > base_num) {
+            uint32_t rel = (uint32_t)(D.n_lo - base_num);
+            bitlo = (int)((rel / 30u) * 8u) + c_firstbit[rel % 30u];
+        }
+        if (D.n_hi < base_num + 30ull * D.len - 1ull) {
+            uint32_t rel = (uint32_t)(D.n_hi - base_num);
+            bithi = (int)((rel / 30u) * 8u) + c_lastbit[rel % 30u];
+        }
+        PairCtx pc{hist, mq, &a, base_num, D.job};
+        const uint32_t per = L4 / BLOCK;
+        const uint32_t i0 = (uint32_t)tid * per;
+        uint32_t cnt = 0, c1 = 0, m30[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t prev = 0, prevbit = 0, first = 0;
+        for (uint32_t i = i0; i < i0 + per; i++) {
+            uint32_t w = bm[i];
+            const int lo = (int)(32u * i);
+            if (lo < bitlo) w &= (bitlo - lo >= 32) ? 0u : (0xFFFFFFFFu << (bitlo - lo));
+            if (lo + 31 > bithi)
+                w &= (bithi < lo) ? 0u
+                     : (bithi - lo >= 31 ? 0xFFFFFFFFu : ((2u << (bithi - lo)) - 1u));
+            if (!w) continue;
+            cnt += __popc(w);
+            // bytes 0 and 2 of a word have even k (w0 % 4 == 0):
+            // p%4 == 1 for residues {1,13,17,29} at even k, {7,11,19,23} at odd k
+            c1 += __popc(w & 0x66996699u);
+#pragma unroll
+            for (int o = 0; o < 8; o++) m30[o] += __popc(w & (0x01010101u << o));
+            do {
+                const uint32_t b = __builtin_ctz(w);
+                w &= w - 1u;
+                const uint32_t off = 30u * (4u * i + (b >> 3)) + c_w30[b & 7u];
+                if (prev) pair_in_window(pc, prev - 1u, off, prevbit);
+                else first = off + 1u;
+                prev = off + 1u;
+                prevbit = b & 7u;
+            } while (w);
+        }
+        tfirst[tid] = first;
+        tlast[tid] = prev;
+
+        // per-window counts: wave reduction, then one LDS atomic per wave
+        {
+            uint32_t v[10] = {cnt, c1, m30[0], m30[1], m30[2], m30[3],
+                              m30[4], m30[5], m30[6], m30[7]};
+#pragma unroll
+            for (int k = 0; k < 10; k++) {
+                uint32_t x = v[k];
+#pragma unroll
+                for (int s = 32; s >= 1; s >>= 1) x += __shfl_xor(x, s);
+                if (lane == 0 && x) atomicAdd(&s_cnt[k], (unsigned long long)x);
+            }
+        }
+        __syncthreads();
+
+        // pairs that cross thread runs
+        if (first) {
+            for (int s = tid - 1; s >= 0; s--) {
+                const uint32_t l = tlast[s];
+                if (l) {
+                    const uint32_t pb = l - 1u;
+                    pair_in_window(pc, pb, first - 1u, (uint32_t)c_residx[pb % 30u]);
+                    break;
+                }
+            }
+        }
+        if (tid == 0) {
+            SegRec rc;
+            rc.count = s_cnt[0];
+            rc.first = 0;
+            rc.last = 0;
+            if (rc.count) {
+                for (int s = 0; s < BLOCK; s++)
+                    if (tfirst[s]) { rc.first = base_num + tfirst[s] - 1u; break; }
+                for (int s = BLOCK - 1; s >= 0; s--)
+                    if (tlast[s]) { rc.last = base_num + tlast[s] - 1u; break; }
+                JobAcc* acc = &a.acc[D.job];
+                atomicAdd(&acc->count, s_cnt[0]);
+                atomicAdd(&acc->c1, s_cnt[1]);
+                for (int o = 0; o < 8; o++) atomicAdd(&acc->mod30[o], s_cnt[2 + o]);
+                atomicMin(&acc->first, rc.first);
+                atomicMax(&acc->last, rc.last);
+            }
+            a.rec[d] = rc;
+        }
+        __syncthreads();
+    }
+    flush(cur_job);
+}
+
+// Joins the last prime of one window with the first prime of the next
+// non-empty window of the same job (the host boundary gap of the
+// reference, src/voss_v7_live.cu:551-562, generalised to every window
+// edge).  Such a pair counts in-segment only when both windows carry the
+// same logical segment.
+__global__ void stitch_kernel(SieveArgs a) {
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= a.ndesc) return;
+    const SegRec rd = a.rec[d];
+    if (rd.count == 0) return;
+    const uint32_t job = a.desc[d].job;
+    int e = (int)d - 1;
+    for (; e >= 0 && a.desc[e].job == job; e--)
+        if (a.rec[e].count) break;
+    if (e < 0 || a.desc[e].job != job) return;
+    const unsigned long long p = a.rec[e].last;
+    const unsigned long long g64 = rd.first - p;
+    JobAcc* acc = &a.acc[job];
+    if (g64 >= 2ull * kHistBins) {
+        atomicOr(&acc->error, 1ull);
+        return;
+    }
+    const uint32_t g = (uint32_t)g64;
+    atomicAdd(&acc->hist[g >> 1], 1ull);
+    if (a.desc[e].lseg != a.desc[d].lseg) return;
+    if (g < 100u) {
+        const uint32_t o = (uint32_t)c_residx[p % 30ull];
+        atomicAdd(&acc->modq[c_q6row[o] * kModqBins + (g >> 1)], 1ull);
+        atomicAdd(&acc->modq[(2u + o) * kModqBins + (g >> 1)], 1ull);
+    } else if (g > 100u) {
+        if (g >= (uint32_t)kLargeAlways || p <= a.lg_thresh[g >> 1]) {
+            unsigned long long slot = atomicAdd(a.large_n, 1ull);
+            atomicAdd(&acc->large_count, 1ull);
+            if (slot < a.large_cap) {
+                LargeRaw r;
+                r.p_before = p;
+                r.gap = g;
+                r.job = job;
+                a.large[slot] = r;
+            }
+        }
+    }
+}
+
+__global__ void init_acc_kernel(JobAcc* acc, uint32_t njobs) {
+    const uint32_t job = blockIdx.x;
+    if (job >= njobs) return;
+    JobAcc* a = &acc[job];
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(a);
+    const uint32_t nw = sizeof(JobAcc) / 8;
+    for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) w[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) a->first = ~0ull;
+}
+
+// ------------------------------------------------------------------ host
+
+static bool g_const_ready[64];
+
+hipError_t upload_sieve_constants(int device) {
+    if (device >= 0 && device < 64 && g_const_ready[device]) return hipSuccess;
+    int8_t residx[30], firstbit[30], lastbit[30];
+    for (int i = 0; i < 30; i++) residx[i] = -1;
+    for (int o = 0; o < 8; o++) residx[kW30[o]] = (int8_t)o;
+    for (int r = 0; r < 30; r++) {
+        int fb = 8, lb = -1;
+        for (int o = 7; o >= 0; o--) if (kW30[o] >= r) fb = o;
+        for (int o = 0; o < 8; o++) if (kW30[o] <= r) lb = o;
+        firstbit[r] = (int8_t)fb;
+        lastbit[r] = (int8_t)lb;
+    }
+    uint8_t bitidx[8][8];
+    for (int pr = 0; pr < 8; pr++)
+        for (int j = 0; j < 8; j++)
+            bitidx[pr][j] = (uint8_t)residx[(kW30[pr] * kW30[j]) % 30];
+    uint32_t w30[8], q6row[8];
+    static const uint32_t q6[8] = {0, 0, 1, 0, 1, 0, 1, 1};
+    for (int o = 0; o < 8; o++) { w30[o] = (uint32_t)kW30[o]; q6row[o] = q6[o]; }
+    hipError_t e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_residx), residx, sizeof(residx))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_firstbit), firstbit, sizeof(firstbit))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_lastbit), lastbit, sizeof(lastbit))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_bitidx), bitidx, sizeof(bitidx))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_w30), w30, sizeof(w30))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_q6row), q6row, sizeof(q6row))) != hipSuccess) return e;
+    if (device >= 0 && device < 64) g_const_ready[device] = true;
+    return hipSuccess;
+}
+
+hipError_t launch_init_acc(JobAcc* acc, uint32_t njobs, hipStream_t s) {
+    if (njobs == 0) return hipSuccess;
+    hipLaunchKernelGGL(init_acc_kernel, dim3(njobs), dim3(256), 0, s, acc, njobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_sieve(const SieveArgs& a, int window_bytes, uint32_t grid,
+                        hipStream_t s) {
+    if (a.ndesc == 0) return hipSuccess;
+    if (grid > a.ndesc) grid = a.ndesc;
+    switch (window_bytes) {
+        case 131072:
+            hipLaunchKernelGGL((sieve_kernel<131072, 1024>), dim3(grid), dim3(1024), 0, s, a);
+            break;
+        case 65536:
+            hipLaunchKernelGGL((sieve_kernel<65536, 512>), dim3(grid), dim3(512), 0, s, a);
+            break;
+        case 32768:
+            hipLaunchKernelGGL((sieve_kernel<32768, 256>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int sieve_block_threads(int window_bytes) {
+    switch (window_bytes) {
+        case 131072: return 1024;
+        case 65536: return 512;
+        case 32768: return 256;
+        default: return 0;
+    }
+}
+
+hipError_t launch_stitch(const SieveArgs& a, hipStream_t s) {
+    if (a.ndesc == 0) return hipSuccess;
+    const uint32_t blocks = (a.ndesc + 255) / 256;
+    hipLaunchKernelGGL(stitch_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace voss

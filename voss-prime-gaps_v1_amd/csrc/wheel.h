@@ -1,0 +1,76 @@
+// wheel.h -- wheel-30 layout and the device-side records shared by the
+// sieve kernels and the host planner.
+//
+// Bit layout (same meaning as the reference, src/voss_v7_live.cu:37-38,
+// :189): wheel byte k holds the 8 numbers 30k + W30[o], o = 0..7, bit o of
+// the byte.  Bytes are packed little-endian into 32-bit LDS words, so bit
+// (8*j + o) of word i is the number 30*(4i + j) + W30[o].
+#pragma once
+#include <stdint.h>
+
+namespace voss {
+
+constexpr int kW30[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+
+// Device histogram geometry.  Every prime gap below 2^64 is < 1600
+// (maximal gaps are known to 2^64), so even gaps g < 2048 cover all of
+// them; a larger gap raises VOSS_ERANGE instead of being dropped.
+constexpr int kHistBins = 1024;       // bin g/2, g < 2048
+constexpr int kModqBins = 50;         // bin g/2, g < 100
+constexpr int kModqRows = 10;         // q=6 {1,5}, q=30 {1,7,...,29}
+constexpr int kLargeMin = 101;        // merit only for g > 100 (:290)
+constexpr int kLargeAlways = 500;     // LARGE_GAP_THRESHOLD (:33)
+
+// Presieve: primes 7..73 are removed by ANDing precomputed period
+// patterns (6 groups of 3 primes, one table of prod(p) words each).
+constexpr int kPresieveGroups = 6;
+constexpr int kPresievePrimes = 18;   // 7,11,...,73
+constexpr int kPresieveGroupPrimes[kPresieveGroups][3] = {
+    {7, 11, 13}, {17, 19, 23}, {29, 31, 37},
+    {41, 43, 47}, {53, 59, 61}, {67, 71, 73}};
+
+// One unit of sieve work: an LDS window of `len` wheel bytes starting at
+// byte w0 (w0 % 4 == 0, len % 4096 == 0), of which the numbers in
+// [n_lo, n_hi] are counted.  Pairs of primes that lie in two different
+// windows are joined by the stitch kernel; they count as "in-segment"
+// (full rule) only when both windows carry the same lseg.
+struct SegDesc {
+    uint64_t w0;
+    uint64_t n_lo;
+    uint64_t n_hi;
+    uint32_t len;
+    uint32_t nprimes;  // base primes with p*p <= last number of the window
+    uint32_t job;      // output slot (interval index in batch mode)
+    uint32_t lseg;     // logical segment id (plain mode)
+};
+
+// Written by the sieve for every descriptor; read by the stitch kernel.
+struct SegRec {
+    unsigned long long count;
+    unsigned long long first;  // 0 if count == 0
+    unsigned long long last;
+};
+
+// Per-job accumulators in HBM (all unsigned 64-bit, summed with atomics,
+// so the result is independent of scheduling order).
+struct JobAcc {
+    unsigned long long hist[kHistBins];             // all pairs, bin g/2
+    unsigned long long modq[kModqRows * kModqBins]; // in-segment, g < 100
+    unsigned long long count;                       // wheel primes
+    unsigned long long c1;                          // p % 4 == 1
+    unsigned long long mod30[8];                    // by wheel residue
+    unsigned long long first;                       // min wheel prime
+    unsigned long long last;                        // max wheel prime
+    unsigned long long large_count;                 // large gaps emitted
+    unsigned long long error;                       // bit 0: gap >= 2048
+};
+
+// Large-gap candidate emitted by the device; the host turns it into the
+// reference's LargeGap {p_after, gap, merit}.
+struct LargeRaw {
+    unsigned long long p_before;
+    uint32_t gap;
+    uint32_t job;
+};
+
+}  // namespace voss

@@ -1,0 +1,258 @@
+"""ctypes binding of libvoss_amd.so (include/voss/voss.h).
+
+The product path: every call goes to the gfx950 HIP library.  There is no
+CPU fallback -- if the library is missing or no GPU is present, the calls
+raise.  Tests, bench.py and the drop-in driver use this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libvoss_amd.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "voss", "voss.h")
+
+MAX_GAP = 100000
+MODQ_MAX_GAP = 100
+MODQ_ROWS = 10
+Q_SLOTS = 40
+GAP_SEQ_SIZE = 5000000
+
+VOSS_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "ERANGE", -5: "EBUSY",
+          -6: "ENOTFOUND"}
+
+
+class VossError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class LargeGap(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_uint64), ("gap", ctypes.c_int32),
+                ("merit", ctypes.c_float)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("total_primes", ctypes.c_uint64),
+        ("first_prime", ctypes.c_uint64),
+        ("last_prime", ctypes.c_uint64),
+        ("c1", ctypes.c_uint64),
+        ("c3", ctypes.c_uint64),
+        ("modq", ctypes.c_uint64 * Q_SLOTS),
+        ("gaps", ctypes.c_int64 * MAX_GAP),
+        ("modq_gap", ctypes.c_int64 * (MODQ_ROWS * MODQ_MAX_GAP)),
+        ("large_count", ctypes.c_uint64),
+        ("large_capacity", ctypes.c_uint64),
+        ("large", ctypes.POINTER(LargeGap)),
+        ("seconds", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+        ("sieve_ms", ctypes.c_double),
+        ("windows", ctypes.c_uint64),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("window_bytes", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
+                ("tier_a_limit", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+EXPORTS = [
+    "voss_create", "voss_destroy", "voss_last_error", "voss_device_name",
+    "voss_set_tuning", "voss_get_tuning", "voss_run_plain", "voss_sample_plain",
+    "voss_run_interval", "voss_run_batch", "voss_run_batch_plain", "voss_mr_batch",
+    "voss_mr_batch_device", "voss_verify_gaps", "voss_device_alloc", "voss_device_free",
+    "voss_memcpy_h2d", "voss_memcpy_d2h",
+]
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libvoss_amd.so not built: {LIB_PATH} "
+                               "(python voss-prime-gaps_v1_amd/build.py)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, sz, i = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int
+        P = ctypes.POINTER
+        L.voss_create.argtypes = [i, P(vp)]
+        L.voss_destroy.argtypes = [vp]
+        L.voss_destroy.restype = None
+        L.voss_last_error.restype = ctypes.c_char_p
+        L.voss_device_name.argtypes = [vp, ctypes.c_char_p, sz, P(i), P(ctypes.c_double)]
+        L.voss_set_tuning.argtypes = [vp, P(Tuning)]
+        L.voss_get_tuning.argtypes = [vp, P(Tuning)]
+        L.voss_run_plain.argtypes = [vp, u64, u64, P(Result)]
+        L.voss_sample_plain.argtypes = [vp, u64, u64, vp, vp, P(u64)]
+        L.voss_run_interval.argtypes = [vp, u64, u64, P(Result)]
+        L.voss_run_batch.argtypes = [vp, vp, vp, sz, vp]
+        L.voss_run_batch_plain.argtypes = [vp, vp, vp, sz, vp]
+        L.voss_mr_batch.argtypes = [vp, vp, sz, vp]
+        L.voss_mr_batch_device.argtypes = [vp, vp, sz, vp, P(ctypes.c_double)]
+        L.voss_verify_gaps.argtypes = [vp, vp, vp, sz, vp]
+        L.voss_device_alloc.argtypes = [vp, sz, P(vp)]
+        L.voss_device_free.argtypes = [vp, vp]
+        L.voss_memcpy_h2d.argtypes = [vp, vp, vp, sz]
+        L.voss_memcpy_d2h.argtypes = [vp, vp, vp, sz]
+        for name in EXPORTS:
+            if name not in ("voss_destroy", "voss_last_error"):
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != VOSS_OK:
+        raise VossError(rc, lib().voss_last_error().decode(errors="replace"))
+
+
+def result_to_dict(r, n_large=None):
+    n = min(int(r.large_count), int(r.large_capacity)) if n_large is None else n_large
+    return {
+        "total_primes": int(r.total_primes),
+        "first_prime": int(r.first_prime),
+        "last_prime": int(r.last_prime),
+        "c1": int(r.c1),
+        "c3": int(r.c3),
+        "modq": np.ctypeslib.as_array(r.modq).astype(np.uint64).copy(),
+        "gaps": np.ctypeslib.as_array(r.gaps).astype(np.int64).copy(),
+        "modq_gap": np.ctypeslib.as_array(r.modq_gap).astype(np.int64)
+                      .reshape(MODQ_ROWS, MODQ_MAX_GAP).copy(),
+        "large_count": int(r.large_count),
+        "large": [(int(r.large[i].position), int(r.large[i].gap), float(r.large[i].merit))
+                  for i in range(n)],
+        "seconds": float(r.seconds),
+        "device_ms": float(r.device_ms),
+        "sieve_ms": float(r.sieve_ms),
+        "windows": int(r.windows),
+    }
+
+
+def _new_result(large_cap):
+    r = Result()
+    buf = None
+    if large_cap:
+        buf = (LargeGap * large_cap)()
+        r.large = ctypes.cast(buf, ctypes.POINTER(LargeGap))
+        r.large_capacity = large_cap
+    return r, buf
+
+
+class Engine:
+    """One GPU context (voss_ctx)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().voss_create(device, ctypes.byref(self._h)))
+        name = ctypes.create_string_buffer(256)
+        cus = ctypes.c_int()
+        gb = ctypes.c_double()
+        _check(lib().voss_device_name(self._h, name, 256, ctypes.byref(cus), ctypes.byref(gb)))
+        self.name, self.num_cus, self.total_gb = name.value.decode(), cus.value, gb.value
+
+    def close(self):
+        if self._h:
+            lib().voss_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().voss_get_tuning(self._h, ctypes.byref(t)))
+        return {"window_bytes": t.window_bytes, "wg_per_cu": t.wg_per_cu,
+                "tier_a_limit": t.tier_a_limit}
+
+    def set_tuning(self, window_bytes=0, wg_per_cu=0, tier_a_limit=0):
+        t = Tuning(window_bytes, wg_per_cu, tier_a_limit, 0)
+        _check(lib().voss_set_tuning(self._h, ctypes.byref(t)))
+
+    def run_plain(self, N, seg_num, large_cap=1 << 20):
+        r, buf = _new_result(large_cap)
+        _check(lib().voss_run_plain(self._h, N, seg_num, ctypes.byref(r)))
+        d = result_to_dict(r)
+        del buf
+        return d
+
+    def run_interval(self, lo, hi, large_cap=1 << 16):
+        r, buf = _new_result(large_cap)
+        _check(lib().voss_run_interval(self._h, lo, hi, ctypes.byref(r)))
+        d = result_to_dict(r)
+        del buf
+        return d
+
+    def _batch(self, fn, lo, hi, large_cap):
+        lo = np.ascontiguousarray(lo, dtype=np.uint64)
+        hi = np.ascontiguousarray(hi, dtype=np.uint64)
+        n = lo.shape[0]
+        res = (Result * n)()
+        bufs = []
+        for k in range(n):
+            if large_cap:
+                b = (LargeGap * large_cap)()
+                bufs.append(b)
+                res[k].large = ctypes.cast(b, ctypes.POINTER(LargeGap))
+                res[k].large_capacity = large_cap
+        _check(fn(self._h, lo.ctypes.data, hi.ctypes.data, n, ctypes.cast(res, ctypes.c_void_p)))
+        return [result_to_dict(res[k]) for k in range(n)]
+
+    def run_batch(self, lo, hi, large_cap=64):
+        return self._batch(lib().voss_run_batch, lo, hi, large_cap)
+
+    def run_batch_plain(self, lo, hi, large_cap=64):
+        return self._batch(lib().voss_run_batch_plain, lo, hi, large_cap)
+
+    def mr(self, ns):
+        ns = np.ascontiguousarray(ns, dtype=np.uint64)
+        out = np.zeros(ns.shape[0], dtype=np.uint8)
+        _check(lib().voss_mr_batch(self._h, ns.ctypes.data, ns.shape[0], out.ctypes.data))
+        return out
+
+    def mr_device(self, ns, repeats=1):
+        """MR on device-resident candidates; returns (verdicts, [kernel ms])."""
+        ns = np.ascontiguousarray(ns, dtype=np.uint64)
+        n = ns.shape[0]
+        dn, dv = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().voss_device_alloc(self._h, max(n * 8, 8), ctypes.byref(dn)))
+        _check(lib().voss_device_alloc(self._h, max(n, 8), ctypes.byref(dv)))
+        try:
+            _check(lib().voss_memcpy_h2d(self._h, dn, ns.ctypes.data, n * 8))
+            times = []
+            for _ in range(repeats):
+                ms = ctypes.c_double()
+                _check(lib().voss_mr_batch_device(self._h, dn, n, dv, ctypes.byref(ms)))
+                times.append(ms.value)
+            out = np.zeros(n, dtype=np.uint8)
+            _check(lib().voss_memcpy_d2h(self._h, out.ctypes.data, dv, n))
+        finally:
+            lib().voss_device_free(self._h, dn)
+            lib().voss_device_free(self._h, dv)
+        return out, times
+
+    def verify_gaps(self, p_after, gaps):
+        pa = np.ascontiguousarray(p_after, dtype=np.uint64)
+        g = np.ascontiguousarray(gaps, dtype=np.int32)
+        ok = np.zeros(pa.shape[0], dtype=np.uint8)
+        _check(lib().voss_verify_gaps(self._h, pa.ctypes.data, g.ctypes.data, pa.shape[0],
+                                      ok.ctypes.data))
+        return ok
