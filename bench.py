@@ -81,8 +81,8 @@ def mr_candidates(eng, count, seed=0x564F5353 + 2):
     rnd = rng.integers(2**40, 2**64 - 1, size=count - half, dtype=np.uint64,
                        endpoint=False) | np.uint64(1)
     r = eng.run_interval(10**15, 10**15 + 2 * 10**10, large_cap=1 << 16)
-    ends = np.array([p for p, g, _ in r["large"]] + [p - g for p, g, _ in r["large"]],
-                    dtype=np.uint64)
+    pos = r["large"]["position"].astype(np.uint64)
+    ends = np.concatenate([pos, pos - r["large"]["gap"].astype(np.uint64)])
     pick = ends[rng.integers(0, len(ends), size=half)]
     c = np.concatenate([rnd, pick])
     rng.shuffle(c)

@@ -52,6 +52,17 @@ def build_lib():
     return LIB
 
 
+PROBE = os.path.join(LIB_DIR, "libvoss_probe.so")
+
+
+def build_probe():
+    """LDS roofline microbenchmarks (bench.py reads the measured peak)."""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    _run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(CSRC, "probe.hip"), "-o", PROBE,
+          "-Wl,-rpath,/opt/rocm/lib"])
+    return PROBE
+
+
 def build_cli():
     """C++ host programs over the C ABI (no device code of their own)."""
     os.makedirs(BIN_DIR, exist_ok=True)
@@ -70,9 +81,10 @@ def build_cli():
 
 def build_all():
     build_lib()
+    build_probe()
     build_cli()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"lib": build_lib, "cli": build_cli, "all": build_all}[what]()
+    {"lib": build_lib, "probe": build_probe, "cli": build_cli, "all": build_all}[what]()

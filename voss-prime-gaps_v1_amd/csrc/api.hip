@@ -59,6 +59,28 @@ struct DevBuf {
     }
 };
 
+// Grow-only pinned host staging buffer (hipHostMalloc).
+template <typename T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(n, 64);
+        hipError_t e = hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 constexpr uint32_t kW30Mask[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 
 uint64_t isqrt_u64(uint64_t n) {
@@ -125,6 +147,7 @@ struct voss_ctx {
     uint64_t primes_limit = 0;
     DevBuf<uint32_t> d_primes;
     DevBuf<uint64_t> d_magic;
+    DevBuf<uint32_t> d_perm;
     size_t d_nprimes = 0;
 
     DevBuf<uint32_t> d_pat;
@@ -140,6 +163,12 @@ struct voss_ctx {
     DevBuf<JobAcc> d_acc;
     DevBuf<LargeRaw> d_large;
     DevBuf<unsigned long long> d_large_n;
+
+    HostBuf<SegDesc> h_desc;
+    HostBuf<SegRec> h_rec;
+    HostBuf<JobAcc> h_acc;
+    HostBuf<LargeRaw> h_large;
+    HostBuf<unsigned long long> h_large_n;
 
     DevBuf<uint64_t> d_mr_n;
     DevBuf<uint8_t> d_mr_v;
@@ -164,15 +193,28 @@ int ensure_primes(voss_ctx* c, uint64_t max_number) {
     c->primes = make_base_primes(lim);
     c->primes_limit = lim;
     std::vector<uint64_t> magic(c->primes.size());
-    for (size_t i = 0; i < c->primes.size(); i++) magic[i] = ~0ull / c->primes[i];
+    std::vector<uint32_t> perm(c->primes.size());
+    int residx[30];
+    for (int i = 0; i < 30; i++) residx[i] = -1;
+    for (int o = 0; o < 8; o++) residx[kW30[o]] = o;
+    for (size_t i = 0; i < c->primes.size(); i++) {
+        const uint32_t p = c->primes[i];
+        magic[i] = ~0ull / p;
+        uint32_t w = 0;
+        for (int j = 0; j < 8; j++)
+            w |= (uint32_t)residx[(uint64_t)p * (uint64_t)kW30[j] % 30] << (3 * j);
+        perm[i] = w;
+    }
     const size_t n = std::max<size_t>(c->primes.size(), 1);
     HIPCHK(c->d_primes.reserve(n));
     HIPCHK(c->d_magic.reserve(n));
+    HIPCHK(c->d_perm.reserve(n));
     if (!c->primes.empty()) {
         HIPCHK(hipMemcpy(c->d_primes.p, c->primes.data(), c->primes.size() * 4,
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->d_magic.p, magic.data(), magic.size() * 8,
                          hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_perm.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
     }
     c->d_nprimes = c->primes.size();
     return VOSS_OK;
@@ -190,15 +232,16 @@ uint32_t primes_upto(const voss_ctx* c, uint64_t max_number) {
 int build_patterns(voss_ctx* c) {
     std::vector<uint32_t> all;
     for (int g = 0; g < kPresieveGroups; g++) {
-        const uint32_t* q = (const uint32_t*)kPresieveGroupPrimes[g];
-        const uint32_t P = q[0] * q[1] * q[2];
+        const int* q = kPresieveGroupPrimes[g];
+        const uint32_t P = (uint32_t)(q[0] * q[1] * q[2]);
         c->pat_off[g] = (uint32_t)all.size();
         c->pat_len[g] = P;
         std::vector<uint8_t> bytes(4 * (size_t)P, 0xFF);
         for (int k = 0; k < 3; k++) {
-            const uint32_t p = q[k];
-            // byte k' of the period: numbers 30k'+W30[o]; crossing every
-            // multiple of p (the prime itself is restored in window 0)
+            const uint32_t p = (uint32_t)q[k];
+            if (p == 1) continue;
+            // crossing every multiple of p (the prime itself is restored in
+            // window 0)
             for (int o = 0; o < 8; o++) {
                 // first byte b with 30b + W30[o] = 0 mod p
                 uint32_t b0 = 0;
@@ -217,6 +260,7 @@ int build_patterns(voss_ctx* c) {
     for (int g = 0; g < kPresieveGroups; g++)
         for (int k = 0; k < 3; k++) {
             const uint32_t p = (uint32_t)kPresieveGroupPrimes[g][k];
+            if (p == 1) continue;
             int o = -1;
             for (int i = 0; i < 8; i++) if ((uint32_t)kW30[i] == p % 30) o = i;
             restore |= 1u << ((p / 30) * 8 + o);
@@ -294,16 +338,35 @@ void add_acc(JobHost& J, const JobAcc& A) {
     J.large_count += A.large_count;
 }
 
-bool lg_less(const voss_large_gap& a, const voss_large_gap& b) {
-    if (a.merit != b.merit) return a.merit > b.merit;
-    return a.position < b.position;
+// Canonical order of the large-gap list: merit descending, then position
+// ascending (the reference sorts by merit alone with an unstable sort,
+// src/voss_v7_live.cu:813, so ties have no defined order there).  Merits
+// are positive floats, whose bit patterns order like the values, so one
+// 128-bit key (inverted merit bits : position) sorts both at once.
+void sort_large(std::vector<voss_large_gap>& v) {
+    struct K {
+        unsigned __int128 key;
+        int32_t gap;
+        float merit;
+    };
+    std::vector<K> k(v.size());
+    for (size_t i = 0; i < v.size(); i++) {
+        uint32_t mb;
+        std::memcpy(&mb, &v[i].merit, 4);
+        k[i].key = ((unsigned __int128)(0xFFFFFFFFu - mb) << 64) | v[i].position;
+        k[i].gap = v[i].gap;
+        k[i].merit = v[i].merit;
+    }
+    std::sort(k.begin(), k.end(), [](const K& a, const K& b) { return a.key < b.key; });
+    for (size_t i = 0; i < v.size(); i++)
+        v[i] = {(uint64_t)k[i].key, k[i].gap, k[i].merit};
 }
 
 void write_result(JobHost& J, voss_result* out) {
     std::memcpy(out->gaps, J.gaps.data(), sizeof(out->gaps));
     std::memcpy(out->modq_gap, J.modq_gap.data(), sizeof(out->modq_gap));
     out->large_count = J.large_count;
-    std::sort(J.large.begin(), J.large.end(), lg_less);
+    sort_large(J.large);
     if (out->large && out->large_capacity) {
         const size_t n = std::min<size_t>(J.large.size(), out->large_capacity);
         std::memcpy(out->large, J.large.data(), n * sizeof(voss_large_gap));
@@ -324,12 +387,16 @@ struct Plan {
     uint32_t njobs = 0;
 };
 
-// Windows of the grid w0 = k*W covering numbers [a, b] (a >= 7).
+// Windows of the grid w0 = k*W covering numbers [a, b] (a >= 7).  The
+// base-prime count of consecutive windows only grows, so it is advanced
+// with one pointer instead of a search per window.
 void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job,
                 uint32_t lseg, uint32_t W, uint64_t grid_origin) {
     if (a < 7) a = 7;
     if (a > b) return;
     const uint64_t ka = (a / 30 - grid_origin) / W, kb = (b / 30 - grid_origin) / W;
+    size_t np = 0;
+    const size_t nall = c->primes.size();
     for (uint64_t k = ka; k <= kb; k++) {
         const uint64_t w0 = grid_origin + k * W;
         const uint64_t lo = std::max<uint64_t>(a, 30 * w0);
@@ -342,7 +409,10 @@ void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job
         d.n_lo = lo;
         d.n_hi = hi;
         d.len = (uint32_t)std::min<uint64_t>(need, W);
-        d.nprimes = primes_upto(c, 30 * (w0 + d.len) - 1);
+        const uint64_t top = 30 * (w0 + d.len) - 1;
+        if (np == 0) np = primes_upto(c, top);
+        while (np < nall && (uint64_t)c->primes[np] * c->primes[np] <= top) np++;
+        d.nprimes = (uint32_t)np;
         d.job = job;
         d.lseg = lseg;
         P.desc.push_back(d);
@@ -360,9 +430,11 @@ SieveArgs make_args(voss_ctx* c, uint32_t ndesc, unsigned long long large_cap) {
     a.restore0 = c->restore0;
     a.primes = c->d_primes.p;
     a.magic = c->d_magic.p;
+    a.perm = c->d_perm.p;
     const int block = sieve_block_threads((int)c->tune.window_bytes);
+    a.pat = c->d_pat.p;
     for (int g = 0; g < kPresieveGroups; g++) {
-        a.pat[g] = c->d_pat.p + c->pat_off[g];
+        a.pat_off[g] = c->pat_off[g];
         a.pat_len[g] = c->pat_len[g];
         a.pat_step[g] = (uint32_t)block % c->pat_len[g];
     }
@@ -400,16 +472,18 @@ int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
     std::vector<uint64_t> carry_last(P.njobs, 0);
     std::vector<uint32_t> carry_lseg(P.njobs, 0);
     std::vector<uint8_t> carry_has(P.njobs, 0);
-    std::vector<SegRec> rec;
-    std::vector<JobAcc> acc(P.njobs);
-    std::vector<LargeRaw> raw;
+    HIPCHK(c->h_acc.reserve(P.njobs));
+    HIPCHK(c->h_large_n.reserve(1));
     float total_ms = 0, sieve_ms = 0;
 
     for (size_t off = 0; off < P.desc.size(); off += kMaxDescPerLaunch) {
         const uint32_t nd = (uint32_t)std::min(kMaxDescPerLaunch, P.desc.size() - off);
         HIPCHK(c->d_desc.reserve(nd));
         HIPCHK(c->d_rec.reserve(nd));
-        HIPCHK(hipMemcpyAsync(c->d_desc.p, P.desc.data() + off, nd * sizeof(SegDesc),
+        HIPCHK(c->h_desc.reserve(nd));
+        HIPCHK(c->h_rec.reserve(nd));
+        std::memcpy(c->h_desc.p, P.desc.data() + off, nd * sizeof(SegDesc));
+        HIPCHK(hipMemcpyAsync(c->d_desc.p, c->h_desc.p, nd * sizeof(SegDesc),
                               hipMemcpyHostToDevice, c->stream));
         HIPCHK(launch_init_acc(c->d_acc.p, P.njobs, c->stream));
         HIPCHK(hipMemsetAsync(c->d_large_n.p, 0, sizeof(unsigned long long), c->stream));
@@ -419,12 +493,11 @@ int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
         HIPCHK(hipEventRecord(c->ev2, c->stream));
         HIPCHK(launch_stitch(a, c->stream));
         HIPCHK(hipEventRecord(c->ev1, c->stream));
-        HIPCHK(hipMemcpyAsync(acc.data(), c->d_acc.p, P.njobs * sizeof(JobAcc),
+        HIPCHK(hipMemcpyAsync(c->h_acc.p, c->d_acc.p, P.njobs * sizeof(JobAcc),
                               hipMemcpyDeviceToHost, c->stream));
-        unsigned long long nlarge = 0;
-        HIPCHK(hipMemcpyAsync(&nlarge, c->d_large_n.p, 8, hipMemcpyDeviceToHost, c->stream));
-        rec.resize(nd);
-        HIPCHK(hipMemcpyAsync(rec.data(), c->d_rec.p, nd * sizeof(SegRec),
+        HIPCHK(hipMemcpyAsync(c->h_large_n.p, c->d_large_n.p, 8, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_rec.p, c->d_rec.p, nd * sizeof(SegRec),
                               hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         float ms = 0, sms = 0;
@@ -432,18 +505,25 @@ int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
         HIPCHK(hipEventElapsedTime(&sms, c->ev0, c->ev2));
         total_ms += ms;
         sieve_ms += sms;
-        const uint64_t nkeep = std::min<uint64_t>(nlarge, lcap);
-        raw.resize(nkeep);
-        if (nkeep)
-            HIPCHK(hipMemcpy(raw.data(), c->d_large.p, nkeep * sizeof(LargeRaw),
-                             hipMemcpyDeviceToHost));
+        const JobAcc* acc = c->h_acc.p;
+        const SegRec* rec = c->h_rec.p;
+        const uint64_t nkeep = std::min<uint64_t>(c->h_large_n.p[0], lcap);
+        HIPCHK(c->h_large.reserve(nkeep));
+        if (nkeep) {
+            HIPCHK(hipMemcpyAsync(c->h_large.p, c->d_large.p, nkeep * sizeof(LargeRaw),
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        const LargeRaw* raw = c->h_large.p;
         for (uint32_t j = 0; j < P.njobs; j++) {
             if (acc[j].error) return fail(VOSS_ERANGE, "prime gap >= 2048 (beyond the device histogram)");
             add_acc(jobs[j], acc[j]);
         }
-        for (const LargeRaw& r : raw)
+        for (uint64_t k = 0; k < nkeep; k++) {
+            const LargeRaw& r = raw[k];
             jobs[r.job].large.push_back({r.p_before + r.gap, (int32_t)r.gap,
                                          merit_of(r.p_before, r.gap)});
+        }
         // launch boundary: pair the previous launch's last prime of a job
         // with this launch's first non-empty window of the same job
         std::vector<uint8_t> seen(P.njobs, 0);
@@ -543,7 +623,7 @@ int voss_create(int device, voss_ctx** out) {
         if (e != hipSuccess) rc = hip_fail(e, "hipGetDeviceProperties");
     }
     if (rc == VOSS_OK) {
-        c->name = prop.name;
+        c->name = prop.name[0] ? prop.name : prop.gcnArchName;
         c->num_cus = prop.multiProcessorCount;
         c->total_gb = prop.totalGlobalMem / 1e9;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -570,6 +650,7 @@ void voss_destroy(voss_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->d_primes.release();
     c->d_magic.release();
+    c->d_perm.release();
     c->d_pat.release();
     c->d_lg_thresh.release();
     c->d_desc.release();
@@ -577,6 +658,11 @@ void voss_destroy(voss_ctx* c) {
     c->d_acc.release();
     c->d_large.release();
     c->d_large_n.release();
+    c->h_desc.release();
+    c->h_rec.release();
+    c->h_acc.release();
+    c->h_large.release();
+    c->h_large_n.release();
     c->d_mr_n.release();
     c->d_mr_v.release();
     c->d_surv.release();

@@ -1,0 +1,88 @@
+// probe.hip -- LDS roofline microbenchmarks for gfx950 (used by bench.py to
+// state the measured peak beside the published one).  Every kernel keeps
+// its traffic inside one workgroup's LDS; the only global write is one
+// word per thread so the work cannot be optimised away.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace {
+
+constexpr int kWords = 32768;  // 128 KB of LDS per workgroup
+
+// Streaming 16-byte reads, conflict-free (consecutive lanes, consecutive
+// 16-byte slots).
+__global__ __launch_bounds__(1024) void lds_read_b128(uint32_t iters, uint32_t stride,
+                                                     uint32_t* out) {
+    __shared__ uint4 buf[kWords / 4];
+    for (int i = threadIdx.x; i < kWords / 4; i += blockDim.x)
+        buf[i] = make_uint4(i, i * 3u, i * 5u, i * 7u);
+    __syncthreads();
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    uint32_t idx = threadIdx.x;
+    for (uint32_t it = 0; it < iters; it++) {
+        const uint4 v = buf[idx];
+        acc.x ^= v.x; acc.y += v.y; acc.z ^= v.z; acc.w += v.w;
+        idx = (idx + stride + (acc.x & 1u)) & (kWords / 4 - 1);
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = acc.x ^ acc.y ^ acc.z ^ acc.w;
+}
+
+// ds_and_b32 at pseudo-random word addresses (the sieve's marking pattern).
+__global__ __launch_bounds__(1024) void lds_and_random(uint32_t iters, uint32_t* out) {
+    __shared__ uint32_t buf[kWords];
+    for (int i = threadIdx.x; i < kWords; i += blockDim.x) buf[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t x = threadIdx.x * 2654435761u + blockIdx.x;
+    for (uint32_t it = 0; it < iters; it++) {
+        x = x * 1664525u + 1013904223u;
+        atomicAnd(&buf[x >> 17], ~(1u << (x & 31u)));
+    }
+    __syncthreads();
+    out[blockIdx.x * blockDim.x + threadIdx.x] = buf[threadIdx.x];
+}
+
+// ds_and_b32 at consecutive word addresses (no bank conflicts).
+__global__ __launch_bounds__(1024) void lds_and_linear(uint32_t iters, uint32_t* out) {
+    __shared__ uint32_t buf[kWords];
+    for (int i = threadIdx.x; i < kWords; i += blockDim.x) buf[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t idx = threadIdx.x;
+    for (uint32_t it = 0; it < iters; it++) {
+        atomicAnd(&buf[idx], ~(1u << (it & 31u)));
+        idx = (idx + 1024u) & (kWords - 1);
+    }
+    __syncthreads();
+    out[blockIdx.x * blockDim.x + threadIdx.x] = buf[threadIdx.x];
+}
+
+}  // namespace
+
+// kind 0: read b128 -> bytes/s; 1: ds_and random -> ops/s; 2: ds_and linear.
+extern "C" int voss_probe_lds(int kind, int num_cus, double* rate) {
+    const uint32_t iters = kind == 0 ? 4096 : 8192;
+    const uint32_t grid = (uint32_t)num_cus;
+    uint32_t* out = nullptr;
+    if (hipMalloc((void**)&out, (size_t)grid * 1024 * 4) != hipSuccess) return -1;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    float best = 1e30f;
+    for (int rep = 0; rep < 4; rep++) {
+        (void)hipEventRecord(e0, 0);
+        if (kind == 0)
+            hipLaunchKernelGGL(lds_read_b128, dim3(grid), dim3(1024), 0, 0, iters, 1024u, out);
+        else if (kind == 1) hipLaunchKernelGGL(lds_and_random, dim3(grid), dim3(1024), 0, 0, iters, out);
+        else hipLaunchKernelGGL(lds_and_linear, dim3(grid), dim3(1024), 0, 0, iters, out);
+        (void)hipEventRecord(e1, 0);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0 && ms < best) best = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    const double ops = (double)grid * 1024.0 * iters;
+    *rate = (kind == 0 ? ops * 16.0 : ops) / (best * 1e-3);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -8,8 +8,8 @@
 //
 // Design (one workgroup = one LDS window at a time, grid-stride over the
 // window descriptors):
-//   phase 0  the window's bitmap is initialised in LDS from six presieve
-//            pattern tables (primes 7..73), one AND per table per word;
+//   phase 0  the window's bitmap is initialised from 13 presieve pattern
+//            tables for primes 7..73 held in LDS, one AND per table per word;
 //   phase 1  every other base prime p with p*p <= window end crosses off
 //            its 8 residue classes with ds_and_b32.  Primes below
 //            tier_a_end are marked by a whole wave (8 lanes per class,
@@ -31,12 +31,18 @@
 
 namespace voss {
 
-__constant__ uint8_t c_bitidx[8][8];  // [idx(p%30)][j] -> bit of p*W30[j]
 __constant__ int8_t c_residx[30];     // residue -> wheel bit, -1 if none
 __constant__ int8_t c_firstbit[30];   // smallest o with W30[o] >= r
 __constant__ int8_t c_lastbit[30];    // largest o with W30[o] <= r, or -1
-__constant__ uint32_t c_w30[8];
-__constant__ uint32_t c_q6row[8];     // W30_TO_Q6_IDX (src/voss_v7_live.cu:30)
+
+// W30[o] as eight bytes of one 64-bit constant, and W30_TO_Q6_IDX
+// (src/voss_v7_live.cu:30) as the bits of 0xD4: no table loads in the
+// per-prime path.
+constexpr unsigned long long kW30Bytes = 0x1D1713110D0B0701ull;
+__device__ __forceinline__ uint32_t w30_of(uint32_t o) {
+    return (uint32_t)(kW30Bytes >> (8u * o)) & 0xFFu;
+}
+__device__ __forceinline__ uint32_t q6row_of(uint32_t o) { return (0xD4u >> o) & 1u; }
 
 // x mod p for any 64-bit x: m = floor(2^64 / p) (Barrett, one correction).
 __device__ __forceinline__ uint32_t mod_u64(uint64_t x, uint32_t p,
@@ -51,43 +57,67 @@ __device__ __forceinline__ void clear_bit(uint32_t* bm, uint32_t b,
     atomicAnd(&bm[b >> 2], ~(1u << (((b & 3u) << 3) | bit)));
 }
 
+__device__ __forceinline__ uint32_t clear_mask(uint32_t b, uint32_t bit) {
+    return ~(1u << (((b & 3u) << 3) | bit));
+}
+
+// Crosses bytes b, b+p, b+2p, ... < len of one residue class.  p is odd, so
+// (b & 3) repeats with period 4 along the class: the four word masks are
+// computed once and the loop body is one address and one ds_and per mark.
+__device__ __forceinline__ void cross_class(uint32_t* bm, uint32_t b, uint32_t p,
+                                            uint32_t bit, uint32_t len) {
+    const uint32_t m0 = clear_mask(b, bit), m1 = clear_mask(b + p, bit);
+    const uint32_t m2 = clear_mask(b + 2 * p, bit), m3 = clear_mask(b + 3 * p, bit);
+    const uint32_t p4 = 4 * p;
+    for (; b + 3 * p < len; b += p4) {
+        atomicAnd(&bm[b >> 2], m0);
+        atomicAnd(&bm[(b + p) >> 2], m1);
+        atomicAnd(&bm[(b + 2 * p) >> 2], m2);
+        atomicAnd(&bm[(b + 3 * p) >> 2], m3);
+    }
+    if (b < len) atomicAnd(&bm[b >> 2], m0);
+    if (b + p < len) atomicAnd(&bm[(b + p) >> 2], m1);
+    if (b + 2 * p < len) atomicAnd(&bm[(b + 2 * p) >> 2], m2);
+}
+
 // Position of the first multiple of p in residue class j inside the window
 // starting at byte w0 (relative byte index, may be >= len).  The multiple
 // p*1 (the prime itself) is skipped; smaller composite multiples p*m with
 // 7 <= m < p may be crossed, which changes nothing.
 __device__ __forceinline__ uint32_t first_pos(uint32_t p, uint32_t r,
                                               int j, uint64_t w0) {
-    uint32_t off = (uint32_t)(((uint64_t)p * c_w30[j]) / 30u);
+    uint32_t off = (uint32_t)(((uint64_t)p * w30_of((uint32_t)j)) / 30u);
     uint32_t pos = off >= r ? off - r : off + p - r;
     if (j == 0 && w0 <= (uint64_t)off) pos += p;
     return pos;
 }
 
+// Thresholds of the merit rule for the even gaps 102..500 (bins 51..250)
+// live in LDS: lgt[h - 51].
+constexpr int kLgtFirst = 51, kLgtCount = 200;
+
 struct PairCtx {
     uint32_t* hist;
     uint32_t* mq;
+    const unsigned long long* lgt;
     const SieveArgs* a;
     unsigned long long base_num;
     uint32_t job;
+    uint32_t g_maybe;  // below it no gap of this window can be large
 };
 
-// One consecutive pair inside a window: offsets from base_num.
-__device__ __forceinline__ void pair_in_window(const PairCtx& c,
-                                               uint32_t pb, uint32_t pa,
-                                               uint32_t obit) {
-    uint32_t g = pa - pb;
-    if (g < 2u * kHistBins) {
-        atomicAdd(&c.hist[g >> 1], 1u);
-    } else {
+// A consecutive pair inside a window with gap g >= 100 (pb = offset of
+// p_before from base_num).  Gaps below 100 only touch the q=30 row
+// (mq[64*row + g/2]); the q=6 rows and the main histogram follow at flush.
+__device__ __noinline__ void pair_wide(const PairCtx& c, uint32_t pb, uint32_t g) {
+    if (g >= 2u * kHistBins) {
         atomicOr(&c.a->acc[c.job].error, 1ull);
         return;
     }
-    if (g < 100u) {
-        atomicAdd(&c.mq[c_q6row[obit] * kModqBins + (g >> 1)], 1u);
-        atomicAdd(&c.mq[(2u + obit) * kModqBins + (g >> 1)], 1u);
-    } else if (g > 100u) {
+    atomicAdd(&c.hist[g >> 1], 1u);
+    if (g >= c.g_maybe) {
         unsigned long long p_before = c.base_num + pb;
-        if (g >= (uint32_t)kLargeAlways || p_before <= c.a->lg_thresh[g >> 1]) {
+        if (g >= (uint32_t)kLargeAlways || p_before <= c.lgt[(g >> 1) - kLgtFirst]) {
             unsigned long long slot = atomicAdd(c.a->large_n, 1ull);
             atomicAdd(&c.a->acc[c.job].large_count, 1ull);
             if (slot < c.a->large_cap) {
@@ -106,18 +136,23 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
     constexpr int W4 = WIN / 4;
     __shared__ uint32_t bm[W4];
     __shared__ uint32_t hist[kHistBins];
-    __shared__ uint32_t mq[kModqRows * kModqBins];
+    __shared__ uint32_t mq[64 * 8];  // q=30 rows interleaved: mq[8*(g/2) + row]
+    __shared__ uint32_t s_pat[kPresieveWords];
     __shared__ uint32_t tfirst[BLOCK];
     __shared__ uint32_t tlast[BLOCK];
     __shared__ unsigned long long s_cnt[10];
     __shared__ uint32_t s_task;
+    __shared__ uint32_t s_gmaybe;
+    __shared__ unsigned long long s_lgt[kLgtCount];
     __shared__ uint32_t s_patbase[kPresieveGroups];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
 
     for (int i = tid; i < kHistBins; i += BLOCK) hist[i] = 0;
-    for (int i = tid; i < kModqRows * kModqBins; i += BLOCK) mq[i] = 0;
+    for (int i = tid; i < 8 * 64; i += BLOCK) mq[i] = 0;
+    for (int i = tid; i < kPresieveWords; i += BLOCK) s_pat[i] = a.pat[i];
+    for (int i = tid; i < kLgtCount; i += BLOCK) s_lgt[i] = a.lg_thresh[kLgtFirst + i];
     uint32_t cur_job = blockIdx.x < a.ndesc ? a.desc[blockIdx.x].job : 0;
 
     auto flush = [&](uint32_t job) {
@@ -130,11 +165,15 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                 hist[i] = 0;
             }
         }
-        for (int i = tid; i < kModqRows * kModqBins; i += BLOCK) {
-            uint32_t v = mq[i];
-            if (v) {
-                atomicAdd(&acc->modq[i], (unsigned long long)v);
-                mq[i] = 0;
+        for (int i = tid; i < 8 * 64; i += BLOCK) {
+            const uint32_t v = mq[i];
+            mq[i] = 0;
+            if (v && i >= 8) {  // bin 0 collects the gap-0 dummies
+                const uint32_t row = (uint32_t)i & 7u, h = (uint32_t)i >> 3;
+                const unsigned long long v64 = v;
+                atomicAdd(&acc->modq[(2u + row) * kModqBins + h], v64);
+                atomicAdd(&acc->modq[q6row_of(row) * kModqBins + h], v64);
+                atomicAdd(&acc->hist[h], v64);
             }
         }
         __syncthreads();
@@ -150,26 +189,30 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
         const uint64_t dw0 = D.w0 >> 2;
         if (tid < kPresieveGroups)
             s_patbase[tid] = (uint32_t)(dw0 % a.pat_len[tid]);
-        if (tid == 0) s_task = 0;
+        if (tid == 0) {
+            s_task = 0;
+            s_gmaybe = (uint32_t)kLargeAlways;
+        }
         if (tid < 10) s_cnt[tid] = 0;
         __syncthreads();
 
-        // ---- phase 0: presieve patterns (primes 7..73)
+        // lgt grows with the gap: the smallest gap whose threshold reaches
+        // the window start bounds the gaps that may be large in this window
+        if (tid < kLgtCount && s_lgt[tid] >= 30ull * D.w0)
+            atomicMin(&s_gmaybe, 2u * (uint32_t)(kLgtFirst + tid));
+
+        // ---- phase 0: presieve patterns (primes 7..73) from LDS tables
         {
             uint32_t idx[kPresieveGroups];
 #pragma unroll
-            for (int g = 0; g < kPresieveGroups; g++) {
-                uint32_t P = a.pat_len[g];
-                uint32_t v = s_patbase[g] + (uint32_t)tid;
-                while (v >= P) v -= P;
-                idx[g] = v;
-            }
+            for (int g = 0; g < kPresieveGroups; g++)
+                idx[g] = (s_patbase[g] + (uint32_t)tid) % a.pat_len[g];
             for (uint32_t i = tid; i < L4; i += BLOCK) {
                 uint32_t w = 0xFFFFFFFFu;
 #pragma unroll
                 for (int g = 0; g < kPresieveGroups; g++) {
-                    w &= a.pat[g][idx[g]];
-                    uint32_t v = idx[g] + a.pat_step[g];
+                    w &= s_pat[a.pat_off[g] + idx[g]];
+                    const uint32_t v = idx[g] + a.pat_step[g];
                     idx[g] = v >= a.pat_len[g] ? v - a.pat_len[g] : v;
                 }
                 if (i == 0 && D.w0 == 0) w = (w | a.restore0) & ~1u;
@@ -197,21 +240,29 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     const uint32_t r = mod_u64(D.w0, p, a.magic[pi]);
                     const int j = lane >> 3;
                     const uint32_t s = lane & 7;
-                    const uint32_t bit = c_bitidx[c_residx[p % 30u]][j];
+                    const uint32_t bit = (a.perm[pi] >> (3 * j)) & 7u;
                     uint32_t b = first_pos(p, r, j, D.w0) + s * p;
-                    const uint32_t stride = 8u * p;
-                    for (; b < len; b += stride) clear_bit(bm, b, bit);
+                    const uint32_t stride = 8u * p;  // = 0 mod 4: one mask
+                    const uint32_t m = clear_mask(b, bit);
+                    for (; b < len; b += stride) atomicAnd(&bm[b >> 2], m);
                 } else {
                     const uint32_t pi = a.tier_a_end + (t - nA) * 64 + lane;
                     if (pi < np) {
                         const uint32_t p = a.primes[pi];
                         const uint32_t r = mod_u64(D.w0, p, a.magic[pi]);
-                        const int pr = c_residx[p % 30u];
+                        const uint32_t perm = a.perm[pi];
+                        if (p >= len) {
+                            // at most one multiple per class in the window
+#pragma unroll
+                            for (int j = 0; j < 8; j++) {
+                                const uint32_t b = first_pos(p, r, j, D.w0);
+                                if (b < len) clear_bit(bm, b, (perm >> (3 * j)) & 7u);
+                            }
+                        } else {
 #pragma unroll 1
-                        for (int j = 0; j < 8; j++) {
-                            const uint32_t bit = c_bitidx[pr][j];
-                            for (uint32_t b = first_pos(p, r, j, D.w0); b < len; b += p)
-                                clear_bit(bm, b, bit);
+                            for (int j = 0; j < 8; j++)
+                                cross_class(bm, first_pos(p, r, j, D.w0), p,
+                                            (perm >> (3 * j)) & 7u, len);
                         }
                     }
                 }
@@ -230,34 +281,59 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             uint32_t rel = (uint32_t)(D.n_hi - base_num);
             bithi = (int)((rel / 30u) * 8u) + c_lastbit[rel % 30u];
         }
-        PairCtx pc{hist, mq, &a, base_num, D.job};
+        PairCtx pc{hist, mq, s_lgt, &a, base_num, D.job, s_gmaybe};
         const uint32_t per = L4 / BLOCK;
         const uint32_t i0 = (uint32_t)tid * per;
+        const uint32_t iend = i0 + per;
         uint32_t cnt = 0, c1 = 0, m30[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        uint32_t prev = 0, prevbit = 0, first = 0;
-        for (uint32_t i = i0; i < i0 + per; i++) {
-            uint32_t w = bm[i];
+        // pre-pass: clip the run to [n_lo, n_hi] and count by residue
+        for (uint32_t i = i0; i < iend; i++) {
+            const uint32_t w0 = bm[i];
+            uint32_t w = w0;
             const int lo = (int)(32u * i);
             if (lo < bitlo) w &= (bitlo - lo >= 32) ? 0u : (0xFFFFFFFFu << (bitlo - lo));
             if (lo + 31 > bithi)
                 w &= (bithi < lo) ? 0u
                      : (bithi - lo >= 31 ? 0xFFFFFFFFu : ((2u << (bithi - lo)) - 1u));
-            if (!w) continue;
+            if (w != w0) bm[i] = w;
             cnt += __popc(w);
             // bytes 0 and 2 of a word have even k (w0 % 4 == 0):
             // p%4 == 1 for residues {1,13,17,29} at even k, {7,11,19,23} at odd k
             c1 += __popc(w & 0x66996699u);
 #pragma unroll
             for (int o = 0; o < 8; o++) m30[o] += __popc(w & (0x01010101u << o));
-            do {
-                const uint32_t b = __builtin_ctz(w);
-                w &= w - 1u;
-                const uint32_t off = 30u * (4u * i + (b >> 3)) + c_w30[b & 7u];
-                if (prev) pair_in_window(pc, prev - 1u, off, prevbit);
-                else first = off + 1u;
-                prev = off + 1u;
-                prevbit = b & 7u;
-            } while (w);
+        }
+        // walk: primes of the run in increasing order, word by word.  The
+        // first prime of the run pairs with itself (gap 0), which lands in
+        // the unused bin 0 instead of taking a branch.
+        uint32_t first = 0, prev = 0;
+        {
+            uint32_t prevoff = 0, rowb = 0;  // rowb = 4 * wheel bit of p_before
+            char* const mqb = (char*)mq;
+            for (uint32_t i = i0; i < iend; i++) {
+                uint32_t w = bm[i];
+                if (!w) continue;
+                const uint32_t base = 120u * i;
+                do {
+                    const uint32_t b = __builtin_ctz(w);
+                    w &= w - 1u;
+                    // 30*(b>>3) + W30[b&7]; v_perm picks byte (b&7) of W30 and
+                    // selector bytes 0x0C give zeros above it
+                    const uint32_t w30 = __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u,
+                                                               (b & 7u) | 0x0C0C0C00u);
+                    const uint32_t off = base + 30u * (b >> 3) + w30;
+                    const uint32_t g = prev ? off - prevoff : 0u;
+                    first = first ? first : off + 1u;
+                    // byte address 4*(8*(g/2) + row) = 16*g + 4*row (g even):
+                    // equal gaps of different rows fall in different banks
+                    if (g < 100u) atomicAdd((uint32_t*)(mqb + rowb + 16u * g), 1u);
+                    else pair_wide(pc, prevoff, g);
+                    prev = 1u;
+                    prevoff = off;
+                    rowb = (b & 7u) << 2;
+                } while (w);
+            }
+            prev = prev ? prevoff + 1u : 0u;
         }
         tfirst[tid] = first;
         tlast[tid] = prev;
@@ -282,7 +358,9 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                 const uint32_t l = tlast[s];
                 if (l) {
                     const uint32_t pb = l - 1u;
-                    pair_in_window(pc, pb, first - 1u, (uint32_t)c_residx[pb % 30u]);
+                    const uint32_t g = first - 1u - pb;
+                    if (g < 100u) atomicAdd(&mq[((g >> 1) << 3) + (uint32_t)c_residx[pb % 30u]], 1u);
+                    else pair_wide(pc, pb, g);
                     break;
                 }
             }
@@ -338,7 +416,7 @@ __global__ void stitch_kernel(SieveArgs a) {
     if (a.desc[e].lseg != a.desc[d].lseg) return;
     if (g < 100u) {
         const uint32_t o = (uint32_t)c_residx[p % 30ull];
-        atomicAdd(&acc->modq[c_q6row[o] * kModqBins + (g >> 1)], 1ull);
+        atomicAdd(&acc->modq[q6row_of(o) * kModqBins + (g >> 1)], 1ull);
         atomicAdd(&acc->modq[(2u + o) * kModqBins + (g >> 1)], 1ull);
     } else if (g > 100u) {
         if (g >= (uint32_t)kLargeAlways || p <= a.lg_thresh[g >> 1]) {
@@ -382,20 +460,10 @@ hipError_t upload_sieve_constants(int device) {
         firstbit[r] = (int8_t)fb;
         lastbit[r] = (int8_t)lb;
     }
-    uint8_t bitidx[8][8];
-    for (int pr = 0; pr < 8; pr++)
-        for (int j = 0; j < 8; j++)
-            bitidx[pr][j] = (uint8_t)residx[(kW30[pr] * kW30[j]) % 30];
-    uint32_t w30[8], q6row[8];
-    static const uint32_t q6[8] = {0, 0, 1, 0, 1, 0, 1, 1};
-    for (int o = 0; o < 8; o++) { w30[o] = (uint32_t)kW30[o]; q6row[o] = q6[o]; }
     hipError_t e;
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_residx), residx, sizeof(residx))) != hipSuccess) return e;
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_firstbit), firstbit, sizeof(firstbit))) != hipSuccess) return e;
     if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_lastbit), lastbit, sizeof(lastbit))) != hipSuccess) return e;
-    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_bitidx), bitidx, sizeof(bitidx))) != hipSuccess) return e;
-    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_w30), w30, sizeof(w30))) != hipSuccess) return e;
-    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_q6row), q6row, sizeof(q6row))) != hipSuccess) return e;
     if (device >= 0 && device < 64) g_const_ready[device] = true;
     return hipSuccess;
 }

@@ -14,7 +14,9 @@ struct SieveArgs {
     uint32_t restore0;      // word 0 bits of the presieved primes 7..73
     const uint32_t* primes; // base primes >= 7, ascending
     const uint64_t* magic;  // floor(2^64 / p) per prime
-    const uint32_t* pat[kPresieveGroups];
+    const uint32_t* perm;   // 3 bits per class j: wheel bit of p*W30[j]
+    const uint32_t* pat;                 // all tables, kPresieveWords words
+    uint32_t pat_off[kPresieveGroups];   // table start in pat
     uint32_t pat_len[kPresieveGroups];   // period in words
     uint32_t pat_step[kPresieveGroups];  // block size mod period
     const unsigned long long* lg_thresh; // [kHistBins]: largest p_before

@@ -18,16 +18,22 @@ constexpr int kW30[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 constexpr int kHistBins = 1024;       // bin g/2, g < 2048
 constexpr int kModqBins = 50;         // bin g/2, g < 100
 constexpr int kModqRows = 10;         // q=6 {1,5}, q=30 {1,7,...,29}
+// In LDS only the 8 q=30 rows are kept: the q=6 rows and the main
+// histogram below g=100 are sums of them for pairs inside a window.
 constexpr int kLargeMin = 101;        // merit only for g > 100 (:290)
 constexpr int kLargeAlways = 500;     // LARGE_GAP_THRESHOLD (:33)
 
-// Presieve: primes 7..73 are removed by ANDing precomputed period
-// patterns (6 groups of 3 primes, one table of prod(p) words each).
-constexpr int kPresieveGroups = 6;
+// Presieve: primes 7..73 are removed by ANDing period patterns held in
+// LDS: one table of prod(p) words per group (the word pattern of a prime
+// p repeats every p words).  13 tables, 3652 words = 14.6 KB.
+constexpr int kPresieveGroups = 13;
 constexpr int kPresievePrimes = 18;   // 7,11,...,73
 constexpr int kPresieveGroupPrimes[kPresieveGroups][3] = {
-    {7, 11, 13}, {17, 19, 23}, {29, 31, 37},
-    {41, 43, 47}, {53, 59, 61}, {67, 71, 73}};
+    {7, 11, 13}, {17, 19, 1}, {23, 29, 1}, {31, 37, 1}, {41, 1, 1},
+    {43, 1, 1},  {47, 1, 1},  {53, 1, 1},  {59, 1, 1},  {61, 1, 1},
+    {67, 1, 1},  {71, 1, 1},  {73, 1, 1}};
+constexpr int kPresieveWords = 1001 + 323 + 667 + 1147 + 41 + 43 + 47 + 53 +
+                               59 + 61 + 67 + 71 + 73;
 
 // One unit of sieve work: an LDS window of `len` wheel bytes starting at
 // byte w0 (w0 % 4 == 0, len % 4096 == 0), of which the numbers in

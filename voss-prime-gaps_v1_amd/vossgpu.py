@@ -35,6 +35,17 @@ class LargeGap(ctypes.Structure):
                 ("merit", ctypes.c_float)]
 
 
+LARGE_DTYPE = np.dtype([("position", "<u8"), ("gap", "<i4"), ("merit", "<f4")])
+
+
+def large_array(ptr, n):
+    """View n voss_large_gap records at ptr as a numpy structured array (copy)."""
+    if n == 0:
+        return np.zeros(0, dtype=LARGE_DTYPE)
+    raw = (ctypes.c_char * (16 * n)).from_address(ctypes.addressof(ptr.contents))
+    return np.frombuffer(raw, dtype=LARGE_DTYPE).copy()
+
+
 class Result(ctypes.Structure):
     _fields_ = [
         ("total_primes", ctypes.c_uint64),
@@ -125,8 +136,7 @@ def result_to_dict(r, n_large=None):
         "modq_gap": np.ctypeslib.as_array(r.modq_gap).astype(np.int64)
                       .reshape(MODQ_ROWS, MODQ_MAX_GAP).copy(),
         "large_count": int(r.large_count),
-        "large": [(int(r.large[i].position), int(r.large[i].gap), float(r.large[i].merit))
-                  for i in range(n)],
+        "large": large_array(r.large, n),
         "seconds": float(r.seconds),
         "device_ms": float(r.device_ms),
         "sieve_ms": float(r.sieve_ms),
@@ -187,19 +197,26 @@ class Engine:
         t = Tuning(window_bytes, wg_per_cu, tier_a_limit, 0)
         _check(lib().voss_set_tuning(self._h, ctypes.byref(t)))
 
+    def _result(self, large_cap):
+        """A Result with a large-gap buffer of at least large_cap entries,
+        reused across calls (allocating and zeroing it per call costs more
+        than the host side of a small run)."""
+        if getattr(self, "_res_cap", -1) < large_cap:
+            self._res, self._res_buf = _new_result(large_cap)
+            self._res_cap = large_cap
+        r = self._res
+        r.large_capacity = large_cap
+        return r
+
     def run_plain(self, N, seg_num, large_cap=1 << 20):
-        r, buf = _new_result(large_cap)
+        r = self._result(large_cap)
         _check(lib().voss_run_plain(self._h, N, seg_num, ctypes.byref(r)))
-        d = result_to_dict(r)
-        del buf
-        return d
+        return result_to_dict(r)
 
     def run_interval(self, lo, hi, large_cap=1 << 16):
-        r, buf = _new_result(large_cap)
+        r = self._result(large_cap)
         _check(lib().voss_run_interval(self._h, lo, hi, ctypes.byref(r)))
-        d = result_to_dict(r)
-        del buf
-        return d
+        return result_to_dict(r)
 
     def _batch(self, fn, lo, hi, large_cap):
         lo = np.ascontiguousarray(lo, dtype=np.uint64)
