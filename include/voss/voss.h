@@ -79,6 +79,26 @@ typedef struct {
     uint64_t windows;        /* out: LDS windows sieved */
 } voss_result;
 
+/* Compact result of one interval of a batch.  gaps[h] counts the gap 2h
+ * (h >= 1) and gaps[0] the gap 1 (the pair (2,3)); every prime gap below
+ * 2^64 is below 2048.  modq_gap[row * VOSS_MODQ_BINS + h] counts the gap 2h
+ * of the row's residue class (rows as in voss_result). */
+#define VOSS_GAP_BINS 1024
+#define VOSS_MODQ_BINS 50
+typedef struct {
+    uint64_t lo, hi;
+    uint64_t total_primes;
+    uint64_t first_prime;
+    uint64_t last_prime;
+    uint64_t c1, c3;
+    uint64_t modq[VOSS_Q_SLOTS];
+    uint64_t gaps[VOSS_GAP_BINS];
+    uint64_t modq_gap[VOSS_MODQ_ROWS * VOSS_MODQ_BINS];
+    uint64_t large_count;
+    uint64_t large_capacity;
+    voss_large_gap* large;
+} voss_interval_result;
+
 typedef struct voss_ctx voss_ctx;
 
 /* Tunables chosen by voss_autotune() or by hand; zero fields keep the
@@ -126,15 +146,18 @@ int voss_sample_plain(voss_ctx* ctx, uint64_t N, uint64_t seg_num,
 int voss_run_interval(voss_ctx* ctx, uint64_t lo, uint64_t hi,
                       voss_result* out);
 
-/* Many intervals in one persistent launch (captured in a HIP graph).
- * out[i] receives interval i; out[i].large/large_capacity are honoured. */
+/* Many intervals [lo[i], hi[i]) in one persistent launch.  out[i]
+ * receives interval i (out[i].large / large_capacity are honoured);
+ * stats (optional) receives the batch's wall and device times in
+ * seconds/device_ms/sieve_ms/windows (its other fields are zeroed). */
 int voss_run_batch(voss_ctx* ctx, const uint64_t* lo, const uint64_t* hi,
-                   size_t n, voss_result* out);
+                   size_t n, voss_interval_result* out, voss_result* stats);
 
 /* The same intervals with one launch sequence per interval (the plain
  * per-interval form the batched form is measured against). */
 int voss_run_batch_plain(voss_ctx* ctx, const uint64_t* lo,
-                         const uint64_t* hi, size_t n, voss_result* out);
+                         const uint64_t* hi, size_t n,
+                         voss_interval_result* out, voss_result* stats);
 
 /* ----------------------------------------------------------- Miller-Rabin */
 

@@ -116,8 +116,8 @@ def test_batch_matches_single_and_oracle(engine, oracle):
     lo = [a for a, _ in iv]
     hi = [b for _, b in iv]
     print(f"[gpu] batch of {len(iv)}", flush=True)
-    batch = engine.run_batch(lo, hi, large_cap=4096)
-    single = engine.run_batch_plain(lo, hi, large_cap=4096)
+    batch, _ = engine.run_batch(lo, hi, large_cap=4096)
+    single, _ = engine.run_batch_plain(lo, hi, large_cap=4096)
     for (a, b), rb, rs in zip(iv, batch, single):
         ref = oracle.run_interval(a, b)
         assert_same(rb, ref)
@@ -147,3 +147,22 @@ def test_bad_arguments(engine):
         engine.run_plain(1000, 31)
     with pytest.raises(vossgpu.VossError):
         engine.run_interval(10, 5)
+
+
+def test_interval_1e13_fixture(engine):
+    """[1e13, 1e13 + 1e12): base primes to 3.3e6, 254k windows; every output
+    against the oracle fixture (sha256 over the full large-gap list)."""
+    import hashlib
+    print("[gpu] interval [1e13, 1.1e13)", flush=True)
+    fx = json.load(open(os.path.join(GOLDEN, "oracle_interval_1e13_1e12.json")))
+    r = engine.run_interval(fx["lo"], fx["hi"], large_cap=1 << 22)
+    for k in FIELDS:
+        assert r[k] == fx[k], k
+    assert [int(x) for x in r["modq"]] == fx["modq"]
+    assert {str(g): int(c) for g, c in enumerate(r["gaps"]) if c} == fx["gaps"]
+    assert {f"{row},{g}": int(c) for row in range(10) for g, c in enumerate(r["modq_gap"][row])
+            if c} == fx["modq_gap"]
+    h = hashlib.sha256()
+    for p, g, m in r["large"].tolist():
+        h.update(f"{p},{g},{m:.6f}\n".encode())
+    assert h.hexdigest() == fx["large_sha256"]

@@ -145,9 +145,7 @@ struct voss_ctx {
 
     std::vector<uint32_t> primes;
     uint64_t primes_limit = 0;
-    DevBuf<uint32_t> d_primes;
-    DevBuf<uint64_t> d_magic;
-    DevBuf<uint32_t> d_perm;
+    DevBuf<PrimeRec> d_primes;
     size_t d_nprimes = 0;
 
     DevBuf<uint32_t> d_pat;
@@ -192,30 +190,24 @@ int ensure_primes(voss_ctx* c, uint64_t max_number) {
     lim = std::min<uint64_t>(lim, 0xFFFFFFFFull);
     c->primes = make_base_primes(lim);
     c->primes_limit = lim;
-    std::vector<uint64_t> magic(c->primes.size());
-    std::vector<uint32_t> perm(c->primes.size());
+    std::vector<PrimeRec> rec(c->primes.size());
     int residx[30];
     for (int i = 0; i < 30; i++) residx[i] = -1;
     for (int o = 0; o < 8; o++) residx[kW30[o]] = o;
     for (size_t i = 0; i < c->primes.size(); i++) {
         const uint32_t p = c->primes[i];
-        magic[i] = ~0ull / p;
         uint32_t w = 0;
         for (int j = 0; j < 8; j++)
             w |= (uint32_t)residx[(uint64_t)p * (uint64_t)kW30[j] % 30] << (3 * j);
-        perm[i] = w;
+        rec[i].p = p;
+        rec[i].perm = w;
+        rec[i].magic = ~0ull / p;
     }
     const size_t n = std::max<size_t>(c->primes.size(), 1);
     HIPCHK(c->d_primes.reserve(n));
-    HIPCHK(c->d_magic.reserve(n));
-    HIPCHK(c->d_perm.reserve(n));
-    if (!c->primes.empty()) {
-        HIPCHK(hipMemcpy(c->d_primes.p, c->primes.data(), c->primes.size() * 4,
+    if (!c->primes.empty())
+        HIPCHK(hipMemcpy(c->d_primes.p, rec.data(), rec.size() * sizeof(PrimeRec),
                          hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->d_magic.p, magic.data(), magic.size() * 8,
-                         hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->d_perm.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
-    }
     c->d_nprimes = c->primes.size();
     return VOSS_OK;
 }
@@ -294,40 +286,50 @@ int build_lg_thresh(voss_ctx* c) {
 }
 
 // Host-side accumulation of one job (the device accumulators plus the pairs
-// joined on the host: the head of the range and launch boundaries).
+// joined on the host: the head of the range and launch boundaries), in the
+// compact bin layout: hist2[h] = gap 2h, gap1 = gap 1 (only (2,3)).
 struct JobHost {
-    std::vector<int64_t> gaps;     // [VOSS_MAX_GAP]
-    std::vector<int64_t> modq_gap; // [VOSS_MODQ_ROWS * VOSS_MODQ_MAX_GAP]
+    int64_t gap1 = 0;
+    std::vector<int64_t> hist2;  // [kHistBins]
+    std::vector<int64_t> mq2;    // [kModqRows * kModqBins]
     uint64_t count = 0, c1 = 0, mod30[8] = {};
     uint64_t first = 0, last = 0;
     uint64_t large_count = 0;
+    bool overflow = false;       // a gap >= 2048 (none exists below 2^64)
     std::vector<voss_large_gap> large;
-    JobHost() : gaps(VOSS_MAX_GAP, 0), modq_gap(VOSS_MODQ_ROWS * VOSS_MODQ_MAX_GAP, 0) {}
+    JobHost() : hist2(kHistBins, 0), mq2(kModqRows * kModqBins, 0) {}
 };
 
 void host_pair(JobHost& J, uint64_t p, uint64_t q, bool full) {
     const uint64_t g = q - p;
     if (g == 0) return;
-    if (g < VOSS_MAX_GAP) J.gaps[g]++;
+    if (g == 1) {
+        J.gap1++;
+        return;  // (2,3): no residue row, never large
+    }
+    if (g >= 2ull * kHistBins || (g & 1)) {
+        J.overflow = true;
+        return;
+    }
+    J.hist2[g / 2]++;
     if (!full || p < 7) return;
     if (g < VOSS_MODQ_MAX_GAP) {
         static const int q6[8] = {0, 0, 1, 0, 1, 0, 1, 1};
         int o = -1;
         for (int i = 0; i < 8; i++) if ((uint64_t)kW30[i] == p % 30) o = i;
-        J.modq_gap[q6[o] * VOSS_MODQ_MAX_GAP + g]++;
-        J.modq_gap[(2 + o) * VOSS_MODQ_MAX_GAP + g]++;
+        J.mq2[q6[o] * kModqBins + g / 2]++;
+        J.mq2[(2 + o) * kModqBins + g / 2]++;
     }
-    if (g > 100 && g < VOSS_MAX_GAP && (g >= (uint64_t)kLargeAlways || merit_ge10(p, (uint32_t)g))) {
+    if (g > 100 && (g >= (uint64_t)kLargeAlways || merit_ge10(p, (uint32_t)g))) {
         J.large_count++;
         J.large.push_back({q, (int32_t)g, merit_of(p, (uint32_t)g)});
     }
 }
 
 void add_acc(JobHost& J, const JobAcc& A) {
-    for (int h = 1; h < kHistBins; h++) J.gaps[2 * h] += (int64_t)A.hist[h];
-    for (int r = 0; r < kModqRows; r++)
-        for (int h = 1; h < kModqBins; h++)
-            J.modq_gap[r * VOSS_MODQ_MAX_GAP + 2 * h] += (int64_t)A.modq[r * kModqBins + h];
+    for (int h = 1; h < kHistBins; h++) J.hist2[h] += (int64_t)A.hist[h];
+    for (int i = 0; i < kModqRows * kModqBins; i++)
+        if (i % kModqBins) J.mq2[i] += (int64_t)A.modq[i];
     J.count += A.count;
     J.c1 += A.c1;
     for (int o = 0; o < 8; o++) J.mod30[o] += A.mod30[o];
@@ -362,9 +364,15 @@ void sort_large(std::vector<voss_large_gap>& v) {
         v[i] = {(uint64_t)k[i].key, k[i].gap, k[i].merit};
 }
 
+// Histograms and large gaps into the reference's array layout.
 void write_result(JobHost& J, voss_result* out) {
-    std::memcpy(out->gaps, J.gaps.data(), sizeof(out->gaps));
-    std::memcpy(out->modq_gap, J.modq_gap.data(), sizeof(out->modq_gap));
+    std::memset(out->gaps, 0, sizeof(out->gaps));
+    std::memset(out->modq_gap, 0, sizeof(out->modq_gap));
+    out->gaps[1] = J.gap1;
+    for (int h = 1; h < kHistBins; h++) out->gaps[2 * h] = J.hist2[h];
+    for (int r = 0; r < kModqRows; r++)
+        for (int h = 1; h < kModqBins; h++)
+            out->modq_gap[r * VOSS_MODQ_MAX_GAP + 2 * h] = J.mq2[r * kModqBins + h];
     out->large_count = J.large_count;
     sort_large(J.large);
     if (out->large && out->large_capacity) {
@@ -429,8 +437,6 @@ SieveArgs make_args(voss_ctx* c, uint32_t ndesc, unsigned long long large_cap) {
     if (a.tier_a_end < a.tier_a_begin) a.tier_a_end = a.tier_a_begin;
     a.restore0 = c->restore0;
     a.primes = c->d_primes.p;
-    a.magic = c->d_magic.p;
-    a.perm = c->d_perm.p;
     const int block = sieve_block_threads((int)c->tune.window_bytes);
     a.pat = c->d_pat.p;
     for (int g = 0; g < kPresieveGroups; g++) {
@@ -569,37 +575,73 @@ void clear_result(voss_result* out) {
 
 // Interval bookkeeping shared by the single and batched forms: primes
 // 2, 3, 5 of [lo, hi) precede the wheel primes.
-void finish_interval(JobHost& J, uint64_t lo, uint64_t hi, voss_result* out) {
-    clear_result(out);
-    uint64_t total = 0, first = 0, last = 0;
-    uint64_t c1 = 0, c3 = 0;
+struct IntervalHead {
+    uint64_t total = 0, first = 0, last = 0, c1 = 0, c3 = 0;
     uint64_t modq[VOSS_Q_SLOTS] = {};
+};
+
+IntervalHead finish_interval_common(JobHost& J, uint64_t lo, uint64_t hi) {
+    IntervalHead H;
     static const uint64_t small[3] = {2, 3, 5};
     for (uint64_t p : small) {
         if (p < lo || p >= hi) continue;
-        if ((p & 3) == 1) c1++;
-        if ((p & 3) == 3) c3++;
-        modq[p % 6]++;
-        modq[6 + p % 30]++;
-        if (total) host_pair(J, last, p, true);
-        else first = p;
-        last = p;
-        total++;
+        if ((p & 3) == 1) H.c1++;
+        if ((p & 3) == 3) H.c3++;
+        H.modq[p % 6]++;
+        H.modq[6 + p % 30]++;
+        if (H.total) host_pair(J, H.last, p, true);
+        else H.first = p;
+        H.last = p;
+        H.total++;
     }
     if (J.count) {
-        if (total) host_pair(J, last, J.first, true);
-        else first = J.first;
-        last = J.last;
+        if (H.total) host_pair(J, H.last, J.first, true);
+        else H.first = J.first;
+        H.last = J.last;
     }
-    total += J.count;
-    wheel_modq(J, modq);
-    out->total_primes = total;
-    out->first_prime = first;
-    out->last_prime = last;
-    out->c1 = c1 + J.c1;
-    out->c3 = c3 + (J.count - J.c1);
-    std::memcpy(out->modq, modq, sizeof(modq));
+    H.total += J.count;
+    wheel_modq(J, H.modq);
+    H.c1 += J.c1;
+    H.c3 += J.count - J.c1;
+    return H;
+}
+
+void finish_interval(JobHost& J, uint64_t lo, uint64_t hi, voss_result* out) {
+    clear_result(out);
+    const IntervalHead H = finish_interval_common(J, lo, hi);
+    out->total_primes = H.total;
+    out->first_prime = H.first;
+    out->last_prime = H.last;
+    out->c1 = H.c1;
+    out->c3 = H.c3;
+    std::memcpy(out->modq, H.modq, sizeof(H.modq));
     write_result(J, out);
+}
+
+void finish_interval(JobHost& J, uint64_t lo, uint64_t hi, voss_interval_result* out) {
+    voss_large_gap* lg = out->large;
+    const uint64_t cap = out->large_capacity;
+    std::memset(out, 0, sizeof(*out));
+    out->large = lg;
+    out->large_capacity = cap;
+    out->lo = lo;
+    out->hi = hi;
+    const IntervalHead H = finish_interval_common(J, lo, hi);
+    out->total_primes = H.total;
+    out->first_prime = H.first;
+    out->last_prime = H.last;
+    out->c1 = H.c1;
+    out->c3 = H.c3;
+    std::memcpy(out->modq, H.modq, sizeof(H.modq));
+    out->gaps[0] = (uint64_t)J.gap1;
+    for (int h = 1; h < kHistBins; h++) out->gaps[h] = (uint64_t)J.hist2[h];
+    for (int i = 0; i < kModqRows * kModqBins; i++) out->modq_gap[i] = (uint64_t)J.mq2[i];
+    out->large_count = J.large_count;
+    sort_large(J.large);
+    if (out->large && out->large_capacity) {
+        const size_t n = std::min<size_t>(J.large.size(), out->large_capacity);
+        std::memcpy(out->large, J.large.data(), n * sizeof(voss_large_gap));
+    }
 }
 
 }  // namespace
@@ -649,8 +691,6 @@ void voss_destroy(voss_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->d_primes.release();
-    c->d_magic.release();
-    c->d_perm.release();
     c->d_pat.release();
     c->d_lg_thresh.release();
     c->d_desc.release();
@@ -734,8 +774,9 @@ int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) 
     // (5, first wheel prime) is the reference's host boundary gap of
     // segment 0 (:559-562): main histogram only
     if (J.count) host_pair(J, 5, J.first, false);
-    J.gaps[1] += 1;  // (2,3)  :634
-    J.gaps[2] += 1;  // (3,5)  :635
+    J.gap1 += 1;      // (2,3)  :634
+    J.hist2[1] += 1;  // (3,5)  :635
+    if (J.overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");
     out->total_primes = 3 + J.count;
     out->first_prime = 2;
     out->last_prime = J.count ? J.last : 5;
@@ -771,6 +812,7 @@ int voss_run_interval(voss_ctx* c, uint64_t lo, uint64_t hi, voss_result* out) {
     const uint64_t cap = out->large ? out->large_capacity : 0;
     if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm))) return rc;
     finish_interval(jobs[0], lo, hi, out);
+    if (jobs[0].overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");
     out->seconds = now_s() - t0;
     out->device_ms = tm.device_ms;
     out->sieve_ms = tm.sieve_ms;
@@ -779,14 +821,14 @@ int voss_run_interval(voss_ctx* c, uint64_t lo, uint64_t hi, voss_result* out) {
 }
 
 int voss_run_batch(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
-                   voss_result* out) {
+                   voss_interval_result* out, voss_result* stats) {
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     if (n && (!lo || !hi || !out)) return fail(VOSS_EINVAL, "NULL argument");
     if (n > 0xFFFFFFFFull) return fail(VOSS_EINVAL, "too many intervals");
     int rc;
     uint64_t maxhi = 0, cap = 0;
     for (size_t i = 0; i < n; i++) {
-        if ((rc = check_result(&out[i]))) return rc;
+        if (out[i].large_capacity && !out[i].large) return fail(VOSS_EINVAL, "large buffer is NULL");
         if (hi[i] < lo[i]) return fail(VOSS_EINVAL, "hi < lo");
         if (hi[i] > (1ull << 62)) return fail(VOSS_EINVAL, "hi beyond 2^62");
         maxhi = std::max(maxhi, hi[i]);
@@ -806,24 +848,41 @@ int voss_run_batch(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n
     std::vector<JobHost> jobs;
     ExecTimes tm;
     if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm))) return rc;
-    const double dt = now_s() - t0;
     for (size_t i = 0; i < n; i++) {
         finish_interval(jobs[i], lo[i], hi[i], &out[i]);
-        out[i].seconds = dt;
-        out[i].device_ms = tm.device_ms;
-        out[i].sieve_ms = tm.sieve_ms;
-        out[i].windows = P.desc.size();
+        if (jobs[i].overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");
+    }
+    if (stats) {
+        clear_result(stats);
+        stats->seconds = now_s() - t0;
+        stats->device_ms = tm.device_ms;
+        stats->sieve_ms = tm.sieve_ms;
+        stats->windows = P.desc.size();
     }
     return VOSS_OK;
 }
 
 int voss_run_batch_plain(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
-                         voss_result* out) {
+                         voss_interval_result* out, voss_result* stats) {
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     if (n && (!lo || !hi || !out)) return fail(VOSS_EINVAL, "NULL argument");
+    const double t0 = now_s();
+    double dev = 0, sv = 0;
+    uint64_t win = 0;
     for (size_t i = 0; i < n; i++) {
-        int rc = voss_run_interval(c, lo[i], hi[i], &out[i]);
+        voss_result st;
+        int rc = voss_run_batch(c, lo + i, hi + i, 1, out + i, &st);
         if (rc) return rc;
+        dev += st.device_ms;
+        sv += st.sieve_ms;
+        win += st.windows;
+    }
+    if (stats) {
+        clear_result(stats);
+        stats->seconds = now_s() - t0;
+        stats->device_ms = dev;
+        stats->sieve_ms = sv;
+        stats->windows = win;
     }
     return VOSS_OK;
 }
@@ -896,12 +955,12 @@ int voss_verify_gaps(voss_ctx* c, const uint64_t* p_after, const int32_t* gap, s
         }
     }
     const size_t kChunk = 4096;
-    std::vector<voss_result> res;
+    std::vector<voss_interval_result> res;
     for (size_t off = 0; off < lo.size(); off += kChunk) {
         const size_t m = std::min(kChunk, lo.size() - off);
-        res.assign(m, voss_result());
-        for (auto& r : res) { r.large = nullptr; r.large_capacity = 0; }
-        if ((rc = voss_run_batch(c, lo.data() + off, hi.data() + off, m, res.data()))) return rc;
+        res.assign(m, voss_interval_result());
+        if ((rc = voss_run_batch(c, lo.data() + off, hi.data() + off, m, res.data(), nullptr)))
+            return rc;
         for (size_t k = 0; k < m; k++)
             if (res[k].total_primes != 0) ok[idx[off + k]] = 0;
     }

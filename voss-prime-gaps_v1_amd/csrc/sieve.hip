@@ -86,7 +86,9 @@ __device__ __forceinline__ void cross_class(uint32_t* bm, uint32_t b, uint32_t p
 // 7 <= m < p may be crossed, which changes nothing.
 __device__ __forceinline__ uint32_t first_pos(uint32_t p, uint32_t r,
                                               int j, uint64_t w0) {
-    uint32_t off = (uint32_t)(((uint64_t)p * w30_of((uint32_t)j)) / 30u);
+    // p*W30[j] fits 32 bits for every base prime of numbers below 2^54
+    const uint32_t off = p < 148102320u ? (p * w30_of((uint32_t)j)) / 30u
+                                        : (uint32_t)(((uint64_t)p * w30_of((uint32_t)j)) / 30u);
     uint32_t pos = off >= r ? off - r : off + p - r;
     if (j == 0 && w0 <= (uint64_t)off) pos += p;
     return pos;
@@ -236,11 +238,12 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                 if (t >= ntask) break;
                 if (t < nA) {
                     const uint32_t pi = a.tier_a_begin + t;
-                    const uint32_t p = a.primes[pi];
-                    const uint32_t r = mod_u64(D.w0, p, a.magic[pi]);
+                    const PrimeRec pr = a.primes[pi];
+                    const uint32_t p = pr.p;
+                    const uint32_t r = mod_u64(D.w0, p, pr.magic);
                     const int j = lane >> 3;
                     const uint32_t s = lane & 7;
-                    const uint32_t bit = (a.perm[pi] >> (3 * j)) & 7u;
+                    const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
                     uint32_t b = first_pos(p, r, j, D.w0) + s * p;
                     const uint32_t stride = 8u * p;  // = 0 mod 4: one mask
                     const uint32_t m = clear_mask(b, bit);
@@ -248,9 +251,10 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                 } else {
                     const uint32_t pi = a.tier_a_end + (t - nA) * 64 + lane;
                     if (pi < np) {
-                        const uint32_t p = a.primes[pi];
-                        const uint32_t r = mod_u64(D.w0, p, a.magic[pi]);
-                        const uint32_t perm = a.perm[pi];
+                        const PrimeRec pr = a.primes[pi];
+                        const uint32_t p = pr.p;
+                        const uint32_t r = mod_u64(D.w0, p, pr.magic);
+                        const uint32_t perm = pr.perm;
                         if (p >= len) {
                             // at most one multiple per class in the window
 #pragma unroll

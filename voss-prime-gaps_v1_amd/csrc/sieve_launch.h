@@ -12,9 +12,7 @@ struct SieveArgs {
     uint32_t tier_a_begin;  // first prime index not covered by presieve
     uint32_t tier_a_end;    // first prime index marked one-lane-per-prime
     uint32_t restore0;      // word 0 bits of the presieved primes 7..73
-    const uint32_t* primes; // base primes >= 7, ascending
-    const uint64_t* magic;  // floor(2^64 / p) per prime
-    const uint32_t* perm;   // 3 bits per class j: wheel bit of p*W30[j]
+    const PrimeRec* primes; // base primes >= 7, ascending
     const uint32_t* pat;                 // all tables, kPresieveWords words
     uint32_t pat_off[kPresieveGroups];   // table start in pat
     uint32_t pat_len[kPresieveGroups];   // period in words

@@ -50,6 +50,15 @@ struct SegDesc {
     uint32_t lseg;     // logical segment id (plain mode)
 };
 
+// Per base prime, one 16-byte record read with a single load: the prime,
+// the wheel bit of p*W30[j] for each residue class j (3 bits each) and the
+// Barrett constant floor(2^64 / p).
+struct PrimeRec {
+    uint32_t p;
+    uint32_t perm;
+    unsigned long long magic;
+};
+
 // Written by the sieve for every descriptor; read by the stitch kernel.
 struct SegRec {
     unsigned long long count;

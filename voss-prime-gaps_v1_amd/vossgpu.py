@@ -66,6 +66,53 @@ class Result(ctypes.Structure):
     ]
 
 
+GAP_BINS = 1024
+MODQ_BINS = 50
+
+
+class IntervalResult(ctypes.Structure):
+    """voss_interval_result: compact per-interval result of the batch APIs."""
+    _fields_ = [
+        ("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64),
+        ("total_primes", ctypes.c_uint64),
+        ("first_prime", ctypes.c_uint64),
+        ("last_prime", ctypes.c_uint64),
+        ("c1", ctypes.c_uint64),
+        ("c3", ctypes.c_uint64),
+        ("modq", ctypes.c_uint64 * Q_SLOTS),
+        ("gaps", ctypes.c_uint64 * GAP_BINS),
+        ("modq_gap", ctypes.c_uint64 * (MODQ_ROWS * MODQ_BINS)),
+        ("large_count", ctypes.c_uint64),
+        ("large_capacity", ctypes.c_uint64),
+        ("large", ctypes.POINTER(LargeGap)),
+    ]
+
+
+def interval_to_dict(r):
+    """Expands a compact result to the layout of result_to_dict."""
+    gaps = np.zeros(MAX_GAP, dtype=np.int64)
+    bins = np.ctypeslib.as_array(r.gaps).astype(np.int64)
+    gaps[1] = bins[0]
+    gaps[2:2 * GAP_BINS:2] = bins[1:]
+    mq = np.zeros((MODQ_ROWS, MODQ_MAX_GAP), dtype=np.int64)
+    mb = np.ctypeslib.as_array(r.modq_gap).astype(np.int64).reshape(MODQ_ROWS, MODQ_BINS)
+    mq[:, 0:2 * MODQ_BINS:2] = mb
+    n = min(int(r.large_count), int(r.large_capacity))
+    return {
+        "lo": int(r.lo), "hi": int(r.hi),
+        "total_primes": int(r.total_primes),
+        "first_prime": int(r.first_prime),
+        "last_prime": int(r.last_prime),
+        "c1": int(r.c1),
+        "c3": int(r.c3),
+        "modq": np.ctypeslib.as_array(r.modq).astype(np.uint64).copy(),
+        "gaps": gaps,
+        "modq_gap": mq,
+        "large_count": int(r.large_count),
+        "large": large_array(r.large, n),
+    }
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("window_bytes", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
                 ("tier_a_limit", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
@@ -102,8 +149,8 @@ def lib():
         L.voss_run_plain.argtypes = [vp, u64, u64, P(Result)]
         L.voss_sample_plain.argtypes = [vp, u64, u64, vp, vp, P(u64)]
         L.voss_run_interval.argtypes = [vp, u64, u64, P(Result)]
-        L.voss_run_batch.argtypes = [vp, vp, vp, sz, vp]
-        L.voss_run_batch_plain.argtypes = [vp, vp, vp, sz, vp]
+        L.voss_run_batch.argtypes = [vp, vp, vp, sz, vp, P(Result)]
+        L.voss_run_batch_plain.argtypes = [vp, vp, vp, sz, vp, P(Result)]
         L.voss_mr_batch.argtypes = [vp, vp, sz, vp]
         L.voss_mr_batch_device.argtypes = [vp, vp, sz, vp, P(ctypes.c_double)]
         L.voss_verify_gaps.argtypes = [vp, vp, vp, sz, vp]
@@ -218,26 +265,34 @@ class Engine:
         _check(lib().voss_run_interval(self._h, lo, hi, ctypes.byref(r)))
         return result_to_dict(r)
 
-    def _batch(self, fn, lo, hi, large_cap):
+    def _batch(self, fn, lo, hi, large_cap, raw):
         lo = np.ascontiguousarray(lo, dtype=np.uint64)
         hi = np.ascontiguousarray(hi, dtype=np.uint64)
         n = lo.shape[0]
-        res = (Result * n)()
-        bufs = []
+        res = (IntervalResult * n)()
+        lbuf = (LargeGap * max(large_cap * n, 1))()
+        base = ctypes.addressof(lbuf)
         for k in range(n):
             if large_cap:
-                b = (LargeGap * large_cap)()
-                bufs.append(b)
-                res[k].large = ctypes.cast(b, ctypes.POINTER(LargeGap))
+                res[k].large = ctypes.cast(base + 16 * large_cap * k, ctypes.POINTER(LargeGap))
                 res[k].large_capacity = large_cap
-        _check(fn(self._h, lo.ctypes.data, hi.ctypes.data, n, ctypes.cast(res, ctypes.c_void_p)))
-        return [result_to_dict(res[k]) for k in range(n)]
+        stats = Result()
+        _check(fn(self._h, lo.ctypes.data, hi.ctypes.data, n, ctypes.cast(res, ctypes.c_void_p),
+                  ctypes.byref(stats)))
+        st = {"seconds": stats.seconds, "device_ms": stats.device_ms,
+              "sieve_ms": stats.sieve_ms, "windows": int(stats.windows)}
+        if raw:
+            return res, lbuf, st
+        return [interval_to_dict(res[k]) for k in range(n)], st
 
-    def run_batch(self, lo, hi, large_cap=64):
-        return self._batch(lib().voss_run_batch, lo, hi, large_cap)
+    def run_batch(self, lo, hi, large_cap=64, raw=False):
+        """All intervals in one persistent launch.  Returns (results, stats);
+        raw=True returns the ctypes IntervalResult array instead of dicts."""
+        return self._batch(lib().voss_run_batch, lo, hi, large_cap, raw)
 
-    def run_batch_plain(self, lo, hi, large_cap=64):
-        return self._batch(lib().voss_run_batch_plain, lo, hi, large_cap)
+    def run_batch_plain(self, lo, hi, large_cap=64, raw=False):
+        """One launch sequence per interval (the comparison form)."""
+        return self._batch(lib().voss_run_batch_plain, lo, hi, large_cap, raw)
 
     def mr(self, ns):
         ns = np.ascontiguousarray(ns, dtype=np.uint64)
