@@ -77,6 +77,8 @@ typedef struct {
                                 stitch kernels (HIP events) */
     double sieve_ms;         /* out: device time of the sieve kernel alone */
     uint64_t windows;        /* out: LDS windows sieved */
+    double bucket_ms;        /* out: device time of the large-prime bucket passes */
+    uint64_t bucket_entries; /* out: bucket entries written (large-prime hits) */
 } voss_result;
 
 /* Compact result of one interval of a batch.  gaps[h] counts the gap 2h
@@ -104,10 +106,14 @@ typedef struct voss_ctx voss_ctx;
 /* Tunables chosen by voss_autotune() or by hand; zero fields keep the
  * built-in defaults. */
 typedef struct {
-    uint32_t window_bytes;  /* wheel bytes per LDS window (multiple of 4096) */
+    uint32_t window_bytes;  /* wheel bytes per LDS window: 32768, 65536, 131072 */
     uint32_t wg_per_cu;     /* resident workgroups per CU in the sieve grid */
     uint32_t tier_a_limit;  /* primes below it mark with a whole wave each */
-    uint32_t reserved;
+    uint32_t bucket_limit;  /* primes >= it go through HBM bucket lists
+                               (jobs of >= bucket_min_windows windows);
+                               UINT32_MAX disables buckets */
+    uint32_t bucket_min_windows;
+    uint32_t reserved[3];
 } voss_tuning;
 
 /* ---------------------------------------------------------------- context */

@@ -166,3 +166,42 @@ def test_interval_1e13_fixture(engine):
     for p, g, m in r["large"].tolist():
         h.update(f"{p},{g},{m:.6f}\n".encode())
     assert h.hexdigest() == fx["large_sha256"]
+
+
+@pytest.fixture
+def buckets_forced(engine):
+    """Every prime >= the window length through HBM bucket lists, for every
+    job regardless of its length."""
+    old = engine.tuning()
+    engine.set_tuning(bucket_limit=131072, bucket_min_windows=1)
+    yield engine
+    engine.set_tuning(**{k: v for k, v in old.items()})
+
+
+@pytest.mark.parametrize("lo,hi", [(10**12, 10**12 + 10**9), (10**15, 10**15 + 10**8),
+                                   (10**16 - 3 * 10**7, 10**16 + 10**7), (10**11, 10**11 + 10**5)])
+def test_buckets_interval_matches_oracle(buckets_forced, oracle, lo, hi):
+    print(f"[gpu] buckets interval [{lo},{hi})", flush=True)
+    assert_same(buckets_forced.run_interval(lo, hi), oracle.run_interval(lo, hi))
+
+
+def test_buckets_plain_1e11_fixture(buckets_forced):
+    """25,435 windows in 4 bucket tiles, primes 131,101..316,223 bucketed."""
+    print("[gpu] buckets plain 1e11", flush=True)
+    fx = json.load(open(os.path.join(GOLDEN, "oracle_plain_1e11.json")))
+    r = buckets_forced.run_plain(10**11, 30000000000)
+    for k in FIELDS:
+        assert r[k] == fx[k], k
+    assert {str(g): int(c) for g, c in enumerate(r["gaps"]) if c} == fx["gaps"]
+    assert {f"{row},{g}": int(c) for row in range(10) for g, c in enumerate(r["modq_gap"][row])
+            if c} == fx["modq_gap"]
+
+
+def test_buckets_batch_matches_oracle(buckets_forced, oracle):
+    iv = seeded_intervals(24, 0x564F5353 + 9, lo_min=10**13, lo_max=10**15, lmin=5, lmax=8)
+    lo = [a for a, _ in iv]
+    hi = [b for _, b in iv]
+    print(f"[gpu] buckets batch of {len(iv)}", flush=True)
+    batch, _ = buckets_forced.run_batch(lo, hi, large_cap=4096)
+    for (a, b), rb in zip(iv, batch):
+        assert_same(rb, oracle.run_interval(a, b))

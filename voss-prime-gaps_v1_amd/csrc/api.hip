@@ -141,9 +141,10 @@ struct voss_ctx {
     std::string name;
     int num_cus = 0;
     double total_gb = 0;
-    voss_tuning tune{131072, 1, 1024, 0};
+    voss_tuning tune{131072, 1, 1024, 1u << 21, 4, {0, 0, 0}};
 
     std::vector<uint32_t> primes;
+    std::vector<double> inv_prefix;  // inv_prefix[i] = sum of 1/p over primes[0..i)
     uint64_t primes_limit = 0;
     DevBuf<PrimeRec> d_primes;
     size_t d_nprimes = 0;
@@ -161,6 +162,14 @@ struct voss_ctx {
     DevBuf<JobAcc> d_acc;
     DevBuf<LargeRaw> d_large;
     DevBuf<unsigned long long> d_large_n;
+
+    DevBuf<BucketTask> d_btask;
+    DevBuf<uint32_t> d_bcnt;
+    DevBuf<uint32_t> d_bcur;
+    DevBuf<unsigned long long> d_boff;
+    DevBuf<uint32_t> d_bent;
+    HostBuf<uint32_t> h_bcnt;
+    HostBuf<unsigned long long> h_boff;
 
     HostBuf<SegDesc> h_desc;
     HostBuf<SegRec> h_rec;
@@ -190,6 +199,9 @@ int ensure_primes(voss_ctx* c, uint64_t max_number) {
     lim = std::min<uint64_t>(lim, 0xFFFFFFFFull);
     c->primes = make_base_primes(lim);
     c->primes_limit = lim;
+    c->inv_prefix.assign(c->primes.size() + 1, 0.0);
+    for (size_t i = 0; i < c->primes.size(); i++)
+        c->inv_prefix[i + 1] = c->inv_prefix[i] + 1.0 / c->primes[i];
     std::vector<PrimeRec> rec(c->primes.size());
     int residx[30];
     for (int i = 0; i < 30; i++) residx[i] = -1;
@@ -423,6 +435,8 @@ void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job
         d.nprimes = (uint32_t)np;
         d.job = job;
         d.lseg = lseg;
+        d.bucket = kNoBucket;
+        d.pad = 0;
         P.desc.push_back(d);
     }
 }
@@ -445,6 +459,9 @@ SieveArgs make_args(voss_ctx* c, uint32_t ndesc, unsigned long long large_cap) {
         a.pat_step[g] = (uint32_t)block % c->pat_len[g];
     }
     a.lg_thresh = c->d_lg_thresh.p;
+    a.bucket_off = c->d_boff.p;
+    a.bucket_cnt = c->d_bcnt.p;
+    a.bucket_ent = c->d_bent.p;
     a.rec = c->d_rec.p;
     a.acc = c->d_acc.p;
     a.large = c->d_large.p;
@@ -459,9 +476,139 @@ constexpr size_t kMaxDescPerLaunch = 1u << 20;
 // windows, joining launch boundaries on the host.  jobs[j] receives the
 // accumulated result of job j.  large_cap_total bounds the device list.
 struct ExecTimes {
-    double device_ms = 0;  // sieve + stitch
+    double device_ms = 0;  // buckets + sieve + stitch
     double sieve_ms = 0;   // sieve kernel alone
+    double bucket_ms = 0;  // bucket count + fill kernels
+    uint64_t bucket_entries = 0;
+    uint32_t rounds = 0;
 };
+
+constexpr uint64_t kBucketEntryBudget = 1ull << 30;  // entries per round (4 GB)
+// Primes per bucket task: long tiles give every thread many hits already;
+// short tiles need more primes per task to cover the task's fixed costs.
+uint32_t bucket_chunk(uint32_t nwin) { return nwin >= 256 ? 512u : (nwin >= 32 ? 2048u : 8192u); }
+
+// Splits the plan into rounds (consecutive windows) and, per round, gives
+// the windows of long jobs a bucket for the primes >= bucket_limit.
+struct Round {
+    size_t off, nd;
+    std::vector<SegDesc> desc;     // with bucket / nprimes set for this round
+    std::vector<BucketTask> tasks;
+    uint32_t nbuckets = 0;
+};
+
+void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
+    const uint32_t W = c->tune.window_bytes;
+    const uint32_t bl = c->tune.bucket_limit;
+    const size_t ibl = (size_t)(std::lower_bound(c->primes.begin(), c->primes.end(), bl) -
+                                c->primes.begin());
+    std::vector<uint32_t> job_nwin(P.njobs, 0);
+    for (const SegDesc& d : P.desc) job_nwin[d.job]++;
+    auto bucketed = [&](const SegDesc& d) {
+        return bl != 0xFFFFFFFFu && job_nwin[d.job] >= c->tune.bucket_min_windows &&
+               d.nprimes > ibl;
+    };
+    size_t off = 0;
+    while (off < P.desc.size()) {
+        Round R;
+        R.off = off;
+        double est = 0;
+        size_t end = off;
+        while (end < P.desc.size() && end - off < kMaxDescPerLaunch) {
+            const SegDesc& d = P.desc[end];
+            if (bucketed(d)) {
+                const double e = 8.0 * d.len * (c->inv_prefix[d.nprimes] - c->inv_prefix[ibl]);
+                if (end > off && est + e > 0.7 * (double)kBucketEntryBudget) break;
+                est += e;
+            }
+            end++;
+        }
+        R.nd = end - off;
+        R.desc.assign(P.desc.begin() + off, P.desc.begin() + end);
+        // buckets: consecutive windows of one job with the same w0 share one
+        size_t i = 0;
+        while (i < R.nd) {
+            SegDesc& d = R.desc[i];
+            if (!bucketed(d)) { i++; continue; }
+            // a run of this job's windows (on its grid, step W)
+            size_t j = i;
+            uint32_t nb0 = R.nbuckets;
+            uint64_t prev_w0 = ~0ull;
+            uint32_t np_max = 0;
+            while (j < R.nd && R.desc[j].job == d.job && bucketed(R.desc[j])) {
+                SegDesc& e = R.desc[j];
+                if (e.w0 != prev_w0) {
+                    if (prev_w0 != ~0ull && e.w0 != prev_w0 + W) break;
+                    R.nbuckets++;
+                    prev_w0 = e.w0;
+                }
+                e.bucket = R.nbuckets - 1;
+                np_max = std::max(np_max, e.nprimes);
+                e.nprimes = (uint32_t)ibl;
+                j++;
+            }
+            const uint32_t nwin = R.nbuckets - nb0;
+            for (uint32_t t0 = 0; t0 < nwin; t0 += kBucketTileMax) {
+                const uint32_t tn = std::min<uint32_t>(kBucketTileMax, nwin - t0);
+                const uint32_t chunk = bucket_chunk(tn);
+                for (size_t p0 = ibl; p0 < np_max; p0 += chunk) {
+                    BucketTask T;
+                    T.w0 = d.w0 + (uint64_t)t0 * W;
+                    T.b0 = nb0 + t0;
+                    T.nwin = tn;
+                    T.i0 = (uint32_t)p0;
+                    T.i1 = (uint32_t)std::min<size_t>(np_max, p0 + chunk);
+                    R.tasks.push_back(T);
+                }
+            }
+            i = j;
+        }
+        rounds.push_back(std::move(R));
+        off = end;
+    }
+}
+
+// Bucket count -> host prefix sum -> bucket fill for one round.
+int run_buckets(voss_ctx* c, const Round& R, ExecTimes* tm) {
+    if (R.tasks.empty()) return VOSS_OK;
+    int log2w = 0;
+    while ((1u << log2w) < c->tune.window_bytes) log2w++;
+    const uint32_t grid = (uint32_t)c->num_cus * 2;
+    HIPCHK(c->d_btask.reserve(R.tasks.size()));
+    HIPCHK(c->d_bcnt.reserve(R.nbuckets));
+    HIPCHK(c->d_bcur.reserve(R.nbuckets));
+    HIPCHK(c->d_boff.reserve(R.nbuckets));
+    HIPCHK(c->h_bcnt.reserve(R.nbuckets));
+    HIPCHK(c->h_boff.reserve(R.nbuckets));
+    HIPCHK(hipMemcpyAsync(c->d_btask.p, R.tasks.data(), R.tasks.size() * sizeof(BucketTask),
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_fill_u32(c->d_bcnt.p, R.nbuckets, 0, c->stream));
+    HIPCHK(launch_fill_u32(c->d_bcur.p, R.nbuckets, 0, c->stream));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(launch_bucket_count(c->d_btask.p, (uint32_t)R.tasks.size(), c->d_primes.p,
+                               c->d_bcnt.p, log2w, grid, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_bcnt.p, c->d_bcnt.p, R.nbuckets * 4, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    uint64_t total = 0;
+    for (uint32_t b = 0; b < R.nbuckets; b++) {
+        c->h_boff.p[b] = total;
+        total += c->h_bcnt.p[b];
+    }
+    if (total >= 0xFFFFFFFFull) return fail(VOSS_ENOMEM, "bucket round beyond 2^32 entries");
+    HIPCHK(c->d_bent.reserve(std::max<uint64_t>(total, 1)));
+    HIPCHK(hipMemcpyAsync(c->d_boff.p, c->h_boff.p, R.nbuckets * 8, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(launch_bucket_fill(c->d_btask.p, (uint32_t)R.tasks.size(), c->d_primes.p,
+                              c->d_boff.p, c->d_bcur.p, c->d_bent.p, log2w, grid, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    tm->bucket_ms += ms;
+    tm->bucket_entries += total;
+    return VOSS_OK;
+}
 
 int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
             uint64_t large_cap_total, ExecTimes* times) {
@@ -482,13 +629,19 @@ int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
     HIPCHK(c->h_large_n.reserve(1));
     float total_ms = 0, sieve_ms = 0;
 
-    for (size_t off = 0; off < P.desc.size(); off += kMaxDescPerLaunch) {
-        const uint32_t nd = (uint32_t)std::min(kMaxDescPerLaunch, P.desc.size() - off);
+    std::vector<Round> rounds;
+    make_rounds(c, P, rounds);
+    ExecTimes tmb;
+    for (const Round& R : rounds) {
+        const size_t off = R.off;
+        const uint32_t nd = (uint32_t)R.nd;
+        int brc = run_buckets(c, R, &tmb);
+        if (brc) return brc;
         HIPCHK(c->d_desc.reserve(nd));
         HIPCHK(c->d_rec.reserve(nd));
         HIPCHK(c->h_desc.reserve(nd));
         HIPCHK(c->h_rec.reserve(nd));
-        std::memcpy(c->h_desc.p, P.desc.data() + off, nd * sizeof(SegDesc));
+        std::memcpy(c->h_desc.p, R.desc.data(), nd * sizeof(SegDesc));
         HIPCHK(hipMemcpyAsync(c->d_desc.p, c->h_desc.p, nd * sizeof(SegDesc),
                               hipMemcpyHostToDevice, c->stream));
         HIPCHK(launch_init_acc(c->d_acc.p, P.njobs, c->stream));
@@ -548,8 +701,11 @@ int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
         }
     }
     if (times) {
-        times->device_ms = total_ms;
+        times->device_ms = total_ms + tmb.bucket_ms;
         times->sieve_ms = sieve_ms;
+        times->bucket_ms = tmb.bucket_ms;
+        times->bucket_entries = tmb.bucket_entries;
+        times->rounds = (uint32_t)rounds.size();
     }
     return VOSS_OK;
 }
@@ -698,6 +854,13 @@ void voss_destroy(voss_ctx* c) {
     c->d_acc.release();
     c->d_large.release();
     c->d_large_n.release();
+    c->d_btask.release();
+    c->d_bcnt.release();
+    c->d_bcur.release();
+    c->d_boff.release();
+    c->d_bent.release();
+    c->h_bcnt.release();
+    c->h_boff.release();
     c->h_desc.release();
     c->h_rec.release();
     c->h_acc.release();
@@ -734,6 +897,12 @@ int voss_set_tuning(voss_ctx* c, const voss_tuning* t) {
     }
     if (t->wg_per_cu) n.wg_per_cu = t->wg_per_cu;
     if (t->tier_a_limit) n.tier_a_limit = t->tier_a_limit;
+    if (t->bucket_limit) {
+        if (t->bucket_limit < n.window_bytes && t->bucket_limit != 0xFFFFFFFFu)
+            return fail(VOSS_EINVAL, "bucket_limit must be at least the window length");
+        n.bucket_limit = t->bucket_limit;
+    }
+    if (t->bucket_min_windows) n.bucket_min_windows = t->bucket_min_windows;
     c->tune = n;
     return VOSS_OK;
 }
@@ -788,6 +957,8 @@ int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) 
     out->device_ms = tm.device_ms;
     out->sieve_ms = tm.sieve_ms;
     out->windows = P.desc.size();
+    out->bucket_ms = tm.bucket_ms;
+    out->bucket_entries = tm.bucket_entries;
     return VOSS_OK;
 }
 
@@ -817,6 +988,8 @@ int voss_run_interval(voss_ctx* c, uint64_t lo, uint64_t hi, voss_result* out) {
     out->device_ms = tm.device_ms;
     out->sieve_ms = tm.sieve_ms;
     out->windows = P.desc.size();
+    out->bucket_ms = tm.bucket_ms;
+    out->bucket_entries = tm.bucket_entries;
     return VOSS_OK;
 }
 
@@ -858,6 +1031,8 @@ int voss_run_batch(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n
         stats->device_ms = tm.device_ms;
         stats->sieve_ms = tm.sieve_ms;
         stats->windows = P.desc.size();
+        stats->bucket_ms = tm.bucket_ms;
+        stats->bucket_entries = tm.bucket_entries;
     }
     return VOSS_OK;
 }

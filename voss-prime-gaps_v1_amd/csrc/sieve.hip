@@ -231,6 +231,16 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             const uint32_t nB = np > a.tier_a_end ? (np - a.tier_a_end + 63) / 64 : 0;
             const uint32_t ntask = nA + nB;
             const uint32_t len = D.len;
+            if (D.bucket != kNoBucket) {
+                // large primes: their hits in this window, precomputed
+                const unsigned long long e0 = a.bucket_off[D.bucket];
+                const uint32_t ne = a.bucket_cnt[D.bucket];
+                for (uint32_t k = tid; k < ne; k += BLOCK) {
+                    const uint32_t e = a.bucket_ent[e0 + k];
+                    const uint32_t b = e >> 3;
+                    atomicAnd(&bm[b >> 2], ~(1u << (((b & 3u) << 3) | (e & 7u))));
+                }
+            }
             while (true) {
                 uint32_t t = 0;
                 if (lane == 0) t = atomicAdd(&s_task, 1u);
@@ -448,6 +458,66 @@ __global__ void init_acc_kernel(JobAcc* acc, uint32_t njobs) {
     if (threadIdx.x == 0) a->first = ~0ull;
 }
 
+// Large-prime buckets.  One workgroup per task; thread k handles prime
+// i0 + k/8, residue class k%8, and walks its hits across the task's
+// windows (at most a few per window for primes beyond the window length).
+// COUNT adds the hits per window into bucket_cnt; FILL recounts them in
+// LDS, reserves each window's slots with one global atomic, and writes
+// the entries.  Both passes walk the same hits in the same order.
+template <bool FILL>
+__global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restrict__ tasks,
+                                                      uint32_t ntask,
+                                                      const PrimeRec* __restrict__ primes,
+                                                      uint32_t* __restrict__ bucket_cnt,
+                                                      const unsigned long long* __restrict__ bucket_off,
+                                                      uint32_t* __restrict__ bucket_cursor,
+                                                      uint32_t* __restrict__ entries, int log2w) {
+    __shared__ uint32_t cnt[kBucketTileMax];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wmask = (1u << log2w) - 1u;
+    for (uint32_t t = blockIdx.x; t < ntask; t += gridDim.x) {
+        const BucketTask T = tasks[t];
+        const uint64_t span = (uint64_t)T.nwin << log2w;
+        for (uint32_t i = tid; i < T.nwin; i += blockDim.x) cnt[i] = 0;
+        __syncthreads();
+        const uint32_t nthr = (T.i1 - T.i0) * 8u;
+        for (uint32_t k = tid; k < nthr; k += blockDim.x) {
+            const PrimeRec pr = primes[T.i0 + (k >> 3)];
+            const int j = (int)(k & 7u);
+            const uint32_t r = mod_u64(T.w0, pr.p, pr.magic);
+            for (uint64_t b = first_pos(pr.p, r, j, T.w0); b < span; b += pr.p)
+                atomicAdd(&cnt[(uint32_t)(b >> log2w)], 1u);
+        }
+        __syncthreads();
+        if (!FILL) {
+            for (uint32_t i = tid; i < T.nwin; i += blockDim.x)
+                if (cnt[i]) atomicAdd(&bucket_cnt[T.b0 + i], cnt[i]);
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t i = tid; i < T.nwin; i += blockDim.x)
+            if (cnt[i])
+                cnt[i] = (uint32_t)bucket_off[T.b0 + i] + atomicAdd(&bucket_cursor[T.b0 + i], cnt[i]);
+        __syncthreads();
+        for (uint32_t k = tid; k < nthr; k += blockDim.x) {
+            const PrimeRec pr = primes[T.i0 + (k >> 3)];
+            const int j = (int)(k & 7u);
+            const uint32_t r = mod_u64(T.w0, pr.p, pr.magic);
+            const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
+            for (uint64_t b = first_pos(pr.p, r, j, T.w0); b < span; b += pr.p) {
+                const uint32_t slot = atomicAdd(&cnt[(uint32_t)(b >> log2w)], 1u);
+                entries[slot] = (((uint32_t)b & wmask) << 3) | bit;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
 // ------------------------------------------------------------------ host
 
 static bool g_const_ready[64];
@@ -511,6 +581,37 @@ hipError_t launch_stitch(const SieveArgs& a, hipStream_t s) {
     if (a.ndesc == 0) return hipSuccess;
     const uint32_t blocks = (a.ndesc + 255) / 256;
     hipLaunchKernelGGL(stitch_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace voss
+
+namespace voss {
+
+hipError_t launch_bucket_count(const BucketTask* tasks, uint32_t ntask, const PrimeRec* primes,
+                               uint32_t* bucket_cnt, int log2w, uint32_t grid, hipStream_t s) {
+    if (ntask == 0) return hipSuccess;
+    if (grid > ntask) grid = ntask;
+    hipLaunchKernelGGL(bucket_kernel<false>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes,
+                       bucket_cnt, nullptr, nullptr, nullptr, log2w);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_fill(const BucketTask* tasks, uint32_t ntask, const PrimeRec* primes,
+                              const unsigned long long* bucket_off, uint32_t* bucket_cursor,
+                              uint32_t* entries, int log2w, uint32_t grid, hipStream_t s) {
+    if (ntask == 0) return hipSuccess;
+    if (grid > ntask) grid = ntask;
+    hipLaunchKernelGGL(bucket_kernel<true>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes,
+                       nullptr, bucket_off, bucket_cursor, entries, log2w);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
 

@@ -19,6 +19,9 @@ struct SieveArgs {
     uint32_t pat_step[kPresieveGroups];  // block size mod period
     const unsigned long long* lg_thresh; // [kHistBins]: largest p_before
                                          // with float merit >= 10, per g/2
+    const unsigned long long* bucket_off;  // first entry of each bucket
+    const uint32_t* bucket_cnt;            // entries of each bucket
+    const uint32_t* bucket_ent;            // (offset << 3) | bit
     SegRec* rec;
     JobAcc* acc;
     LargeRaw* large;
@@ -31,6 +34,18 @@ hipError_t launch_init_acc(JobAcc* acc, uint32_t njobs, hipStream_t s);
 hipError_t launch_sieve(const SieveArgs& a, int window_bytes, uint32_t grid,
                         hipStream_t s);
 hipError_t launch_stitch(const SieveArgs& a, hipStream_t s);
+
+// Bucket passes over `ntask` tasks.  count: bucket_cnt[b] += hits.  fill:
+// writes entries using bucket_off and the per-bucket cursors.
+hipError_t launch_bucket_count(const BucketTask* tasks, uint32_t ntask,
+                               const PrimeRec* primes, uint32_t* bucket_cnt,
+                               int log2w, uint32_t grid, hipStream_t s);
+hipError_t launch_bucket_fill(const BucketTask* tasks, uint32_t ntask,
+                              const PrimeRec* primes,
+                              const unsigned long long* bucket_off,
+                              uint32_t* bucket_cursor, uint32_t* entries,
+                              int log2w, uint32_t grid, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 int sieve_block_threads(int window_bytes);
 
 }  // namespace voss

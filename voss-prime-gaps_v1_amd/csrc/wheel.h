@@ -45,10 +45,27 @@ struct SegDesc {
     uint64_t n_lo;
     uint64_t n_hi;
     uint32_t len;
-    uint32_t nprimes;  // base primes with p*p <= last number of the window
+    uint32_t nprimes;  // base primes marked per window: p*p <= last number
+                       // of the window, and p below the bucket limit when
+                       // the window has a bucket
     uint32_t job;      // output slot (interval index in batch mode)
     uint32_t lseg;     // logical segment id (plain mode)
+    uint32_t bucket;   // bucket of this window's large-prime hits, or kNoBucket
+    uint32_t pad;
 };
+constexpr uint32_t kNoBucket = 0xFFFFFFFFu;
+
+// Large-prime bucket fill: one task = the primes [i0, i1) of one job over
+// `nwin` consecutive grid windows starting at byte w0 (buckets b0..).
+// A bucket entry is (byte offset in the window << 3) | wheel bit.
+struct BucketTask {
+    uint64_t w0;
+    uint32_t b0;
+    uint32_t nwin;
+    uint32_t i0;
+    uint32_t i1;
+};
+constexpr int kBucketTileMax = 8192;  // windows per task (LDS counters)
 
 // Per base prime, one 16-byte record read with a single load: the prime,
 // the wheel bit of p*W30[j] for each residue class j (3 bits each) and the

@@ -63,6 +63,8 @@ class Result(ctypes.Structure):
         ("device_ms", ctypes.c_double),
         ("sieve_ms", ctypes.c_double),
         ("windows", ctypes.c_uint64),
+        ("bucket_ms", ctypes.c_double),
+        ("bucket_entries", ctypes.c_uint64),
     ]
 
 
@@ -115,7 +117,8 @@ def interval_to_dict(r):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("window_bytes", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
-                ("tier_a_limit", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("tier_a_limit", ctypes.c_uint32), ("bucket_limit", ctypes.c_uint32),
+                ("bucket_min_windows", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 EXPORTS = [
@@ -188,6 +191,8 @@ def result_to_dict(r, n_large=None):
         "device_ms": float(r.device_ms),
         "sieve_ms": float(r.sieve_ms),
         "windows": int(r.windows),
+        "bucket_ms": float(r.bucket_ms),
+        "bucket_entries": int(r.bucket_entries),
     }
 
 
@@ -238,10 +243,13 @@ class Engine:
         t = Tuning()
         _check(lib().voss_get_tuning(self._h, ctypes.byref(t)))
         return {"window_bytes": t.window_bytes, "wg_per_cu": t.wg_per_cu,
-                "tier_a_limit": t.tier_a_limit}
+                "tier_a_limit": t.tier_a_limit, "bucket_limit": t.bucket_limit,
+                "bucket_min_windows": t.bucket_min_windows}
 
-    def set_tuning(self, window_bytes=0, wg_per_cu=0, tier_a_limit=0):
-        t = Tuning(window_bytes, wg_per_cu, tier_a_limit, 0)
+    def set_tuning(self, window_bytes=0, wg_per_cu=0, tier_a_limit=0, bucket_limit=0,
+                   bucket_min_windows=0):
+        """Zero fields keep the current value."""
+        t = Tuning(window_bytes, wg_per_cu, tier_a_limit, bucket_limit, bucket_min_windows)
         _check(lib().voss_set_tuning(self._h, ctypes.byref(t)))
 
     def _result(self, large_cap):
@@ -280,7 +288,8 @@ class Engine:
         _check(fn(self._h, lo.ctypes.data, hi.ctypes.data, n, ctypes.cast(res, ctypes.c_void_p),
                   ctypes.byref(stats)))
         st = {"seconds": stats.seconds, "device_ms": stats.device_ms,
-              "sieve_ms": stats.sieve_ms, "windows": int(stats.windows)}
+              "sieve_ms": stats.sieve_ms, "windows": int(stats.windows),
+              "bucket_ms": stats.bucket_ms, "bucket_entries": int(stats.bucket_entries)}
         if raw:
             return res, lbuf, st
         return [interval_to_dict(res[k]) for k in range(n)], st
