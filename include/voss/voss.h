@@ -140,6 +140,16 @@ int voss_get_tuning(voss_ctx* ctx, voss_tuning* t);
 int voss_run_plain(voss_ctx* ctx, uint64_t N, uint64_t seg_num,
                    voss_result* out);
 
+/* Logical segments [seg_begin, seg_end) of a plain run, for callers that
+ * run the range in pieces (checkpoint/resume, src/voss_v7_live.cu:599-621):
+ * total_primes counts primes >= 7 of the range, first/last_prime are the
+ * range's own (0 if none), and no head pairs are added (no (2,3), (3,5),
+ * (5,7)).  Joining two consecutive pieces adds the pair (last of the
+ * first, first of the second) to the main histogram only: pieces end on
+ * SEG_NUM boundaries. */
+int voss_run_plain_range(voss_ctx* ctx, uint64_t N, uint64_t seg_num,
+                         uint64_t seg_begin, uint64_t seg_end, voss_result* out);
+
 /* gap_sequence sample of a plain run (sample_gaps_kernel and its host logic,
  * src/voss_v7_live.cu:235-253, :526-547): gap_seq/pos_seq hold
  * VOSS_GAP_SEQ_SIZE entries each, zero where the reference writes nothing.

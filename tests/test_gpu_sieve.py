@@ -205,3 +205,46 @@ def test_buckets_batch_matches_oracle(buckets_forced, oracle):
     batch, _ = buckets_forced.run_batch(lo, hi, large_cap=4096)
     for (a, b), rb in zip(iv, batch):
         assert_same(rb, oracle.run_interval(a, b))
+
+
+@pytest.mark.parametrize("N,SEG", [(10**9, 300000000), (10**8, 30000000000), (123456789, 3932160)])
+def test_gap_sequence_sample_matches_oracle(engine, oracle, N, SEG):
+    """gap_sequence.csv rows (src/voss_v7_live.cu:526-547, :785-803)."""
+    print(f"[gpu] sample N={N} SEG={SEG}", flush=True)
+    gs, ps, rows = engine.sample_plain(N, SEG)
+    ref = oracle.run_plain(N, SEG, sample=True)
+    assert rows == ref["sample_rows"]
+    assert np.array_equal(gs, ref["gap_seq"])
+    assert np.array_equal(ps, ref["pos_seq"])
+
+
+def test_plain_range_pieces_compose(engine):
+    """Pieces of logical segments joined by a main-histogram-only pair equal
+    the whole run (what the CLI does at checkpoints)."""
+    N, SEG = 10**9, 30000000
+    whole = engine.run_plain(N, SEG)
+    nseg = (N + SEG - 1) // SEG
+    cuts = [0, 1, 5, 10, 17, nseg]
+    gaps = np.zeros_like(whole["gaps"])
+    mq = np.zeros_like(whole["modq_gap"])
+    modq = np.zeros_like(whole["modq"])
+    total, last, c1, c3, nlarge = 3, 5, 0, 0, 0
+    for a, b in zip(cuts, cuts[1:]):
+        r = engine.run_plain_range(N, SEG, a, b)
+        if r["total_primes"]:
+            gaps[r["first_prime"] - last] += 1
+            last = r["last_prime"]
+            total += r["total_primes"]
+        gaps += r["gaps"]
+        mq += r["modq_gap"]
+        modq += r["modq"]
+        c1 += r["c1"]
+        c3 += r["c3"]
+        nlarge += r["large_count"]
+    gaps[1] += 1
+    gaps[2] += 1
+    assert np.array_equal(gaps, whole["gaps"])
+    assert np.array_equal(mq, whole["modq_gap"])
+    assert np.array_equal(modq, whole["modq"])
+    assert (total, last, c1, c3, nlarge) == (whole["total_primes"], whole["last_prime"],
+                                             whole["c1"], whole["c3"], whole["large_count"])

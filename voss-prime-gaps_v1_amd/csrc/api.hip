@@ -171,6 +171,13 @@ struct voss_ctx {
     HostBuf<uint32_t> h_bcnt;
     HostBuf<unsigned long long> h_boff;
 
+    // per-window prime counts of the last plain run (gap-sequence sampling)
+    uint64_t last_N = 0, last_seg = 0;
+    uint32_t last_W = 0;
+    std::vector<uint64_t> last_counts;
+    DevBuf<unsigned long long> d_emit;
+    DevBuf<unsigned long long> d_emit_off;
+
     HostBuf<SegDesc> h_desc;
     HostBuf<SegRec> h_rec;
     HostBuf<JobAcc> h_acc;
@@ -462,6 +469,8 @@ SieveArgs make_args(voss_ctx* c, uint32_t ndesc, unsigned long long large_cap) {
     a.bucket_off = c->d_boff.p;
     a.bucket_cnt = c->d_bcnt.p;
     a.bucket_ent = c->d_bent.p;
+    a.emit_out = c->d_emit.p;
+    a.emit_off = c->d_emit_off.p;
     a.rec = c->d_rec.p;
     a.acc = c->d_acc.p;
     a.large = c->d_large.p;
@@ -611,7 +620,8 @@ int run_buckets(voss_ctx* c, const Round& R, ExecTimes* tm) {
 }
 
 int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
-            uint64_t large_cap_total, ExecTimes* times) {
+            uint64_t large_cap_total, ExecTimes* times,
+            std::vector<uint64_t>* counts = nullptr) {
     jobs.assign(P.njobs, JobHost());
     if (P.njobs == 0) return VOSS_OK;
     HIPCHK(c->d_acc.reserve(P.njobs));
@@ -682,6 +692,10 @@ int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
             const LargeRaw& r = raw[k];
             jobs[r.job].large.push_back({r.p_before + r.gap, (int32_t)r.gap,
                                          merit_of(r.p_before, r.gap)});
+        }
+        if (counts) {
+            counts->resize(off + nd);
+            for (uint32_t i = 0; i < nd; i++) (*counts)[off + i] = rec[i].count;
         }
         // launch boundary: pair the previous launch's last prime of a job
         // with this launch's first non-empty window of the same job
@@ -861,6 +875,8 @@ void voss_destroy(voss_ctx* c) {
     c->d_bent.release();
     c->h_bcnt.release();
     c->h_boff.release();
+    c->d_emit.release();
+    c->d_emit_off.release();
     c->h_desc.release();
     c->h_rec.release();
     c->h_acc.release();
@@ -913,32 +929,78 @@ int voss_get_tuning(voss_ctx* c, voss_tuning* t) {
     return VOSS_OK;
 }
 
-int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) {
+}  // extern "C"
+
+namespace {
+
+int plain_args(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) {
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     int rc = check_result(out);
     if (rc) return rc;
     if (N < 5) return fail(VOSS_EINVAL, "N must be >= 5");
-    if (seg_num == 0 || seg_num % 30 != 0) return fail(VOSS_EINVAL, "SEG_NUM must be a positive multiple of 30");
+    if (seg_num == 0 || seg_num % 30 != 0)
+        return fail(VOSS_EINVAL, "SEG_NUM must be a positive multiple of 30");
     if (N > (1ull << 62)) return fail(VOSS_EINVAL, "N beyond 2^62");
-    if ((rc = set_device(c))) return rc;
-    const uint32_t W = c->tune.window_bytes;
-    if ((rc = ensure_primes(c, (N / 30 / W + 1) * W * 30))) return rc;
+    return set_device(c);
+}
 
-    const double t0 = now_s();
+Plan plain_plan(const voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t s0, uint64_t s1) {
     Plan P;
     P.njobs = 1;
-    const uint64_t num_seg = (N + seg_num - 1) / seg_num;  // :342
-    for (uint64_t s = 0; s < num_seg; s++) {
+    const uint32_t W = c->tune.window_bytes;
+    for (uint64_t s = s0; s < s1; s++) {
         const uint64_t a = s * seg_num + 1;
         const uint64_t b = std::min<uint64_t>((s + 1) * seg_num, N);
         plan_range(c, P, a, b, 0, (uint32_t)s, W, 0);
     }
-    std::vector<JobHost> jobs;
-    ExecTimes tm;
-    const uint64_t cap = out->large ? out->large_capacity : 0;
-    if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm))) return rc;
-    JobHost& J = jobs[0];
+    return P;
+}
 
+// Logical segments [s0, s1) of the plain run: primes >= 7 only, no head
+// pairs.  Leaves J with first/last prime of the range.
+int plain_range(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t s0, uint64_t s1,
+                voss_result* out, JobHost& J, ExecTimes& tm, uint64_t* nwin) {
+    const uint32_t W = c->tune.window_bytes;
+    int rc = ensure_primes(c, (N / 30 / W + 1) * W * 30);
+    if (rc) return rc;
+    const Plan P = plain_plan(c, N, seg_num, s0, s1);
+    std::vector<JobHost> jobs;
+    const uint64_t cap = out->large ? out->large_capacity : 0;
+    std::vector<uint64_t> counts;
+    if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm, &counts))) return rc;
+    if (s0 == 0 && s1 == (N + seg_num - 1) / seg_num) {
+        c->last_N = N;
+        c->last_seg = seg_num;
+        c->last_W = W;
+        c->last_counts.swap(counts);
+    }
+    J = std::move(jobs[0]);
+    *nwin = P.desc.size();
+    return VOSS_OK;
+}
+
+void stamp_times(voss_result* out, const ExecTimes& tm, double t0, uint64_t nwin) {
+    out->seconds = now_s() - t0;
+    out->device_ms = tm.device_ms;
+    out->sieve_ms = tm.sieve_ms;
+    out->windows = nwin;
+    out->bucket_ms = tm.bucket_ms;
+    out->bucket_entries = tm.bucket_entries;
+}
+
+}  // namespace
+
+extern "C" {
+
+int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) {
+    int rc = plain_args(c, N, seg_num, out);
+    if (rc) return rc;
+    const double t0 = now_s();
+    JobHost J;
+    ExecTimes tm;
+    uint64_t nwin = 0;
+    const uint64_t num_seg = (N + seg_num - 1) / seg_num;  // :342
+    if ((rc = plain_range(c, N, seg_num, 0, num_seg, out, J, tm, &nwin))) return rc;
     clear_result(out);
     // (5, first wheel prime) is the reference's host boundary gap of
     // segment 0 (:559-562): main histogram only
@@ -953,12 +1015,32 @@ int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) 
     out->c3 = J.count - J.c1;
     wheel_modq(J, out->modq);
     write_result(J, out);
-    out->seconds = now_s() - t0;
-    out->device_ms = tm.device_ms;
-    out->sieve_ms = tm.sieve_ms;
-    out->windows = P.desc.size();
-    out->bucket_ms = tm.bucket_ms;
-    out->bucket_entries = tm.bucket_entries;
+    stamp_times(out, tm, t0, nwin);
+    return VOSS_OK;
+}
+
+int voss_run_plain_range(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t seg_begin,
+                         uint64_t seg_end, voss_result* out) {
+    int rc = plain_args(c, N, seg_num, out);
+    if (rc) return rc;
+    const uint64_t num_seg = (N + seg_num - 1) / seg_num;
+    if (seg_begin > seg_end || seg_end > num_seg)
+        return fail(VOSS_EINVAL, "segment range outside [0, NUM_SEG]");
+    const double t0 = now_s();
+    JobHost J;
+    ExecTimes tm;
+    uint64_t nwin = 0;
+    if ((rc = plain_range(c, N, seg_num, seg_begin, seg_end, out, J, tm, &nwin))) return rc;
+    if (J.overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");
+    clear_result(out);
+    out->total_primes = J.count;
+    out->first_prime = J.count ? J.first : 0;
+    out->last_prime = J.count ? J.last : 0;
+    out->c1 = J.c1;
+    out->c3 = J.count - J.c1;
+    wheel_modq(J, out->modq);
+    write_result(J, out);
+    stamp_times(out, tm, t0, nwin);
     return VOSS_OK;
 }
 
@@ -1062,8 +1144,93 @@ int voss_run_batch_plain(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, si
     return VOSS_OK;
 }
 
-int voss_sample_plain(voss_ctx*, uint64_t, uint64_t, int32_t*, uint64_t*, uint64_t*) {
-    return fail(VOSS_EINVAL, "gap_sequence sampling is not built yet");
+int voss_sample_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, int32_t* gap_seq,
+                      uint64_t* pos_seq, uint64_t* rows_out) {
+    if (!gap_seq || !pos_seq) return fail(VOSS_EINVAL, "NULL buffer");
+    int rc;
+    const uint32_t W = c ? c->tune.window_bytes : 0;
+    if (!c || c->last_N != N || c->last_seg != seg_num || c->last_W != W) {
+        // per-window counts come from a plain run of the same range
+        voss_result* tmp = (voss_result*)std::calloc(1, sizeof(voss_result));
+        if (!tmp) return fail(VOSS_ENOMEM, "host allocation");
+        rc = voss_run_plain(c, N, seg_num, tmp);
+        std::free(tmp);
+        if (rc) return rc;
+    }
+    if ((rc = set_device(c))) return rc;
+    std::memset(gap_seq, 0, sizeof(int32_t) * VOSS_GAP_SEQ_SIZE);
+    std::memset(pos_seq, 0, sizeof(uint64_t) * VOSS_GAP_SEQ_SIZE);
+    const uint64_t num_seg = (N + seg_num - 1) / seg_num;
+    const Plan P = plain_plan(c, N, seg_num, 0, num_seg);
+    const std::vector<uint64_t>& cnt = c->last_counts;
+    if (cnt.size() != P.desc.size()) return fail(VOSS_EINVAL, "stale window counts");
+    const int64_t per_seg = VOSS_GAP_SEQ_SIZE / (int64_t)num_seg;  // :529
+    struct Pick { uint64_t seg, skip, n; size_t emit0; };
+    std::vector<Pick> picks;
+    std::vector<SegDesc> emit_desc;
+    std::vector<unsigned long long> emit_off;
+    unsigned long long total = 0;
+    size_t d = 0;
+    for (uint64_t sg = 0; sg < num_seg; sg++) {
+        const size_t d0 = d;
+        uint64_t n_pos = 0;
+        while (d < P.desc.size() && P.desc[d].lseg == (uint32_t)sg) n_pos += cnt[d++];
+        if (per_seg <= 0 || n_pos <= 1) continue;
+        const int64_t start = (int64_t)(n_pos / 2);
+        const int64_t avail = (int64_t)n_pos - start - 1;
+        const int64_t sample_n = avail < per_seg ? avail : per_seg;
+        if (sample_n <= 1) continue;
+        if ((int64_t)sg * per_seg + sample_n > VOSS_GAP_SEQ_SIZE) continue;
+        // windows holding in-segment prime indices start .. start+sample_n
+        uint64_t before = 0;
+        size_t k = d0;
+        while (before + cnt[k] <= (uint64_t)start) before += cnt[k++];
+        Pick pk{sg, (uint64_t)start - before, (uint64_t)sample_n + 1, emit_off.size()};
+        uint64_t covered = before;
+        const size_t first_emit = emit_desc.size();
+        while (k < d && covered < (uint64_t)start + (uint64_t)sample_n + 1) {
+            emit_desc.push_back(P.desc[k]);
+            emit_off.push_back(total);
+            total += cnt[k];
+            covered += cnt[k];
+            k++;
+        }
+        pk.emit0 = first_emit;
+        picks.push_back(pk);
+    }
+    uint64_t rows = 0;
+    if (!emit_desc.empty()) {
+        const uint32_t nd = (uint32_t)emit_desc.size();
+        HIPCHK(c->d_desc.reserve(nd));
+        HIPCHK(c->d_rec.reserve(nd));
+        HIPCHK(c->d_emit_off.reserve(nd));
+        HIPCHK(c->d_emit.reserve(std::max<unsigned long long>(total, 1)));
+        HIPCHK(c->d_acc.reserve(1));
+        HIPCHK(c->d_large.reserve(1));
+        HIPCHK(c->d_large_n.reserve(1));
+        HIPCHK(hipMemcpyAsync(c->d_desc.p, emit_desc.data(), nd * sizeof(SegDesc),
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_emit_off.p, emit_off.data(), nd * 8, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(launch_init_acc(c->d_acc.p, 1, c->stream));
+        SieveArgs a = make_args(c, nd, 1);
+        HIPCHK(launch_sieve_emit(a, (int)W, (uint32_t)c->num_cus * c->tune.wg_per_cu, c->stream));
+        std::vector<unsigned long long> primes(total);
+        HIPCHK(hipMemcpyAsync(primes.data(), c->d_emit.p, total * 8, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (const Pick& pk : picks) {
+            const unsigned long long* v = primes.data() + emit_off[pk.emit0] + pk.skip;
+            const int64_t off = (int64_t)pk.seg * per_seg;
+            for (uint64_t i = 0; i + 1 < pk.n; i++) {
+                gap_seq[off + (int64_t)i] = (int32_t)(v[i + 1] - v[i]);  // :250
+                pos_seq[off + (int64_t)i] = v[i];                        // :251
+                rows++;
+            }
+        }
+    }
+    if (rows_out) *rows_out = rows;
+    return VOSS_OK;
 }
 
 int voss_mr_batch_device(voss_ctx* c, const uint64_t* d_n, size_t count, uint8_t* d_verdict,

@@ -133,7 +133,7 @@ __device__ __noinline__ void pair_wide(const PairCtx& c, uint32_t pb, uint32_t g
     }
 }
 
-template <int WIN, int BLOCK>
+template <int WIN, int BLOCK, bool EMIT>
 __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
     constexpr int W4 = WIN / 4;
     __shared__ uint32_t bm[W4];
@@ -317,6 +317,23 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
 #pragma unroll
             for (int o = 0; o < 8; o++) m30[o] += __popc(w & (0x01010101u << o));
         }
+        // EMIT: exclusive prefix of the per-thread counts over the block
+        // gives each thread its first output slot (ordered compaction)
+        unsigned long long emit_at = 0;
+        if constexpr (EMIT) {
+            uint32_t incl = cnt;
+#pragma unroll
+            for (int s2 = 1; s2 < 64; s2 <<= 1) {
+                const uint32_t y = __shfl_up(incl, s2);
+                if (lane >= s2) incl += y;
+            }
+            if (lane == 63) tfirst[tid >> 6] = incl;
+            __syncthreads();
+            uint32_t woff = 0;
+            for (int k = 0; k < (tid >> 6); k++) woff += tfirst[k];
+            emit_at = a.emit_off[d] + woff + incl - cnt;
+            __syncthreads();
+        }
         // walk: primes of the run in increasing order, word by word.  The
         // first prime of the run pairs with itself (gap 0), which lands in
         // the unused bin 0 instead of taking a branch.
@@ -336,6 +353,13 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     const uint32_t w30 = __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u,
                                                                (b & 7u) | 0x0C0C0C00u);
                     const uint32_t off = base + 30u * (b >> 3) + w30;
+                    if constexpr (EMIT) {
+                        a.emit_out[emit_at++] = base_num + off;
+                        first = first ? first : off + 1u;
+                        prev = 1u;
+                        prevoff = off;
+                        continue;
+                    }
                     const uint32_t g = prev ? off - prevoff : 0u;
                     first = first ? first : off + 1u;
                     // byte address 4*(8*(g/2) + row) = 16*g + 4*row (g even):
@@ -554,13 +578,33 @@ hipError_t launch_sieve(const SieveArgs& a, int window_bytes, uint32_t grid,
     if (grid > a.ndesc) grid = a.ndesc;
     switch (window_bytes) {
         case 131072:
-            hipLaunchKernelGGL((sieve_kernel<131072, 1024>), dim3(grid), dim3(1024), 0, s, a);
+            hipLaunchKernelGGL((sieve_kernel<131072, 1024, false>), dim3(grid), dim3(1024), 0, s, a);
             break;
         case 65536:
-            hipLaunchKernelGGL((sieve_kernel<65536, 512>), dim3(grid), dim3(512), 0, s, a);
+            hipLaunchKernelGGL((sieve_kernel<65536, 512, false>), dim3(grid), dim3(512), 0, s, a);
             break;
         case 32768:
-            hipLaunchKernelGGL((sieve_kernel<32768, 256>), dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((sieve_kernel<32768, 256, false>), dim3(grid), dim3(256), 0, s, a);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sieve_emit(const SieveArgs& a, int window_bytes, uint32_t grid,
+                             hipStream_t s) {
+    if (a.ndesc == 0) return hipSuccess;
+    if (grid > a.ndesc) grid = a.ndesc;
+    switch (window_bytes) {
+        case 131072:
+            hipLaunchKernelGGL((sieve_kernel<131072, 1024, true>), dim3(grid), dim3(1024), 0, s, a);
+            break;
+        case 65536:
+            hipLaunchKernelGGL((sieve_kernel<65536, 512, true>), dim3(grid), dim3(512), 0, s, a);
+            break;
+        case 32768:
+            hipLaunchKernelGGL((sieve_kernel<32768, 256, true>), dim3(grid), dim3(256), 0, s, a);
             break;
         default:
             return hipErrorInvalidValue;

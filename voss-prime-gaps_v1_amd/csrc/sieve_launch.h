@@ -22,6 +22,8 @@ struct SieveArgs {
     const unsigned long long* bucket_off;  // first entry of each bucket
     const uint32_t* bucket_cnt;            // entries of each bucket
     const uint32_t* bucket_ent;            // (offset << 3) | bit
+    unsigned long long* emit_out;          // EMIT mode: primes, in order
+    const unsigned long long* emit_off;    // EMIT mode: first slot per window
     SegRec* rec;
     JobAcc* acc;
     LargeRaw* large;
@@ -34,6 +36,10 @@ hipError_t launch_init_acc(JobAcc* acc, uint32_t njobs, hipStream_t s);
 hipError_t launch_sieve(const SieveArgs& a, int window_bytes, uint32_t grid,
                         hipStream_t s);
 hipError_t launch_stitch(const SieveArgs& a, hipStream_t s);
+// The sieve in EMIT mode: every counted prime of window d is written, in
+// increasing order, to emit_out[emit_off[d] + i] (no histograms).
+hipError_t launch_sieve_emit(const SieveArgs& a, int window_bytes, uint32_t grid,
+                             hipStream_t s);
 
 // Bucket passes over `ntask` tasks.  count: bucket_cnt[b] += hits.  fill:
 // writes entries using bucket_off and the per-bucket cursors.

@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -28,6 +29,75 @@ namespace {
 
 constexpr int64_t kMaxGap = VOSS_MAX_GAP;
 constexpr int64_t kMaxLarge = 50000000;  // MAX_LARGE_GAPS (:34)
+
+// checkpoint.bin: the reference's CheckpointState (src/voss_v7_live.cu:42-54,
+// 800,064 bytes, native layout) followed by an optional extension with the
+// state the reference does not keep (mod-q counts, per-residue gap
+// histogram, large gaps).  The reference reads the first part only.
+constexpr int64_t kCkptMagic = 0x564F53534F4D4ELL;
+constexpr uint64_t kExtMagic = 0x31444D4153534F56ULL;  // "VOSSAMD1"
+
+struct CheckpointState {
+    int64_t magic;
+    int64_t N;
+    int64_t SEG_NUM;
+    int64_t next_segment;
+    uint64_t last_prime;
+    uint64_t total_primes;
+    unsigned long long class1_total;
+    unsigned long long class3_total;
+    int64_t gaps_total[VOSS_MAX_GAP];
+};
+static_assert(sizeof(CheckpointState) == 800064, "reference checkpoint layout");
+
+struct RunState {
+    CheckpointState ck;
+    bool has_ext = false;
+    uint64_t modq[VOSS_Q_SLOTS] = {};
+    int64_t modq_gap[VOSS_MODQ_ROWS * VOSS_MODQ_MAX_GAP] = {};
+    uint64_t large_count = 0;
+    std::vector<voss_large_gap> large;
+};
+
+bool load_checkpoint(const char* path, int64_t N, int64_t SEG, RunState* st) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    bool ok = std::fread(&st->ck, sizeof(CheckpointState), 1, f) == 1 &&
+              st->ck.magic == kCkptMagic && st->ck.N == N && st->ck.SEG_NUM == SEG;  // :56-68
+    if (ok) {
+        uint64_t m = 0, nl = 0;
+        if (std::fread(&m, 8, 1, f) == 1 && m == kExtMagic &&
+            std::fread(st->modq, sizeof(st->modq), 1, f) == 1 &&
+            std::fread(st->modq_gap, sizeof(st->modq_gap), 1, f) == 1 &&
+            std::fread(&st->large_count, 8, 1, f) == 1 && std::fread(&nl, 8, 1, f) == 1) {
+            st->large.resize(nl);
+            if (nl == 0 || std::fread(st->large.data(), sizeof(voss_large_gap), nl, f) == nl)
+                st->has_ext = true;
+        }
+        if (!st->has_ext) {
+            std::memset(st->modq, 0, sizeof(st->modq));
+            std::memset(st->modq_gap, 0, sizeof(st->modq_gap));
+            st->large_count = 0;
+            st->large.clear();
+        }
+    }
+    std::fclose(f);
+    return ok;
+}
+
+void save_checkpoint(const char* path, const RunState& st) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return;  // the reference ignores a failed checkpoint write (:85-86)
+    const uint64_t nl = st.large.size();
+    std::fwrite(&st.ck, sizeof(CheckpointState), 1, f);
+    std::fwrite(&kExtMagic, 8, 1, f);
+    std::fwrite(st.modq, sizeof(st.modq), 1, f);
+    std::fwrite(st.modq_gap, sizeof(st.modq_gap), 1, f);
+    std::fwrite(&st.large_count, 8, 1, f);
+    std::fwrite(&nl, 8, 1, f);
+    if (nl) std::fwrite(st.large.data(), sizeof(voss_large_gap), nl, f);
+    std::fclose(f);
+}
 
 [[noreturn]] void die(const char* what) {
     std::fprintf(stderr, "VOSS error in %s: %s\n", what, voss_last_error());
@@ -112,7 +182,22 @@ int main(int argc, char** argv) {
     std::printf("GPU: %s (%d CUs, %.1f GB)\n\n", name, cus, gb);
     const int limit = (int)std::sqrt((double)N) + 1;  // :357
     std::printf("Base primes (>=7) up to sqrt(N)=%d\n\n", limit);
-    std::printf(">>> STARTING FRESH <<<\n");
+    static RunState st;
+    std::memset(&st.ck, 0, sizeof(st.ck));
+    int64_t start_seg = 0;
+    if (load_checkpoint("checkpoint.bin", (int64_t)N, (int64_t)SEG_NUM, &st)) {
+        start_seg = st.ck.next_segment;
+        std::printf(">>> RESUMING FROM SEGMENT %lld / %lld <<<\n", (long long)start_seg,
+                    (long long)NUM_SEG);
+    } else {
+        std::memset(&st.ck, 0, sizeof(st.ck));
+        st.ck.magic = kCkptMagic;
+        st.ck.N = (int64_t)N;
+        st.ck.SEG_NUM = (int64_t)SEG_NUM;
+        st.ck.last_prime = 5;    // :404
+        st.ck.total_primes = 3;
+        std::printf(">>> STARTING FRESH <<<\n");
+    }
 
     voss_result* r = (voss_result*)std::calloc(1, sizeof(voss_result));
     const uint64_t cap = std::min<uint64_t>(max_large, (uint64_t)kMaxLarge);
@@ -120,13 +205,71 @@ int main(int argc, char** argv) {
     r->large = large.data();
     r->large_capacity = cap;
 
-    auto t0 = std::chrono::steady_clock::now();
-    if (voss_run_plain(ctx, N, SEG_NUM, r) != VOSS_OK) die("voss_run_plain");
-    const double total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    print_progress(NUM_SEG, NUM_SEG, r->total_primes, total_s, r->last_prime);
+    // logical segments in the reference's checkpoint groups: segment 0,
+    // then up to each (seg+1) % 5 == 0, then the last (:599-602)
+    double total_s = 0, sieve_ms = 0, dev_ms = 0;
+    for (int64_t seg = start_seg; seg < NUM_SEG;) {
+        int64_t end = seg + 1;
+        while (end < NUM_SEG && !(end - 1 == 0 || end % 5 == 0)) end++;
+        if (voss_run_plain_range(ctx, N, SEG_NUM, (uint64_t)seg, (uint64_t)end, r) != VOSS_OK)
+            die("voss_run_plain_range");
+        total_s += r->seconds;
+        sieve_ms += r->sieve_ms;
+        dev_ms += r->device_ms;
+        if (r->total_primes) {
+            // the join of two groups crosses a SEG_NUM boundary: main
+            // histogram only, like the reference's host boundary gap (:559-562)
+            const uint64_t d0 = r->first_prime - st.ck.last_prime;
+            if (d0 > 0 && d0 < (uint64_t)kMaxGap) st.ck.gaps_total[d0]++;
+            st.ck.last_prime = r->last_prime;
+            st.ck.total_primes += r->total_primes;
+        }
+        for (int g = 1; g < kMaxGap; g++) st.ck.gaps_total[g] += r->gaps[g];
+        st.ck.class1_total += r->c1;
+        st.ck.class3_total += r->c3;
+        for (int i = 0; i < VOSS_Q_SLOTS; i++) st.modq[i] += r->modq[i];
+        for (int i = 0; i < VOSS_MODQ_ROWS * VOSS_MODQ_MAX_GAP; i++) st.modq_gap[i] += r->modq_gap[i];
+        st.large_count += r->large_count;
+        const uint64_t nkeep = std::min<uint64_t>(r->large_count, cap);
+        for (uint64_t i = 0; i < nkeep && st.large.size() < cap; i++) st.large.push_back(large[i]);
+        st.ck.next_segment = end;
+        save_checkpoint("checkpoint.bin", st);
+        FILE* stop = std::fopen("stop.txt", "r");
+        if (stop) {  // :603-620
+            std::fclose(stop);
+            std::remove("stop.txt");
+            std::printf("\n>>> STOP-FILE detected at seg %lld — exiting cleanly <<<\n",
+                        (long long)(end - 1));
+            voss_destroy(ctx);
+            return 0;
+        }
+        print_progress(end, NUM_SEG, st.ck.total_primes, total_s, st.ck.last_prime);
+        seg = end;
+    }
     std::printf("\n\n");
     std::remove("checkpoint.bin");
 
+    // results in the reference's arrays
+    std::memset(r, 0, sizeof(*r));
+    std::memcpy(r->gaps, st.ck.gaps_total, sizeof(r->gaps));
+    r->gaps[1] += 1;  // (2,3)  :634
+    r->gaps[2] += 1;  // (3,5)  :635
+    r->total_primes = st.ck.total_primes;
+    r->last_prime = st.ck.last_prime;
+    r->c1 = st.ck.class1_total;
+    r->c3 = st.ck.class3_total;
+    std::memcpy(r->modq, st.modq, sizeof(r->modq));
+    std::memcpy(r->modq_gap, st.modq_gap, sizeof(r->modq_gap));
+    r->large_count = st.large_count;
+    r->seconds = total_s;
+    r->sieve_ms = sieve_ms;
+    r->device_ms = dev_ms;
+    // canonical order: merit desc, position asc
+    std::sort(st.large.begin(), st.large.end(), [](const voss_large_gap& a, const voss_large_gap& b) {
+        if (a.merit != b.merit) return a.merit > b.merit;
+        return a.position < b.position;
+    });
+    large.assign(st.large.begin(), st.large.end());
     int64_t* gaps = r->gaps;
     std::printf("=============================================================\n");
     std::printf("TOTAL TIME: %.2f s (%.2f min)\n", total_s, total_s / 60.0);
@@ -249,13 +392,21 @@ int main(int argc, char** argv) {
         std::vector<uint64_t> ps(VOSS_GAP_SEQ_SIZE, 0);
         uint64_t rows = 0;
         const int rc = voss_sample_plain(ctx, N, SEG_NUM, gs.data(), ps.data(), &rows);
-        if (rc != VOSS_OK && rc != VOSS_EINVAL) die("voss_sample_plain");
+        if (rc != VOSS_OK) die("voss_sample_plain");
+        // after a resume the reference holds samples of this process's
+        // segments only (gap_seq_d is filled per processed segment, :526-547)
+        const int64_t per_seg = VOSS_GAP_SEQ_SIZE / NUM_SEG;
+        for (int64_t i = 0; i < start_seg * per_seg && i < VOSS_GAP_SEQ_SIZE; i++) {
+            gs[i] = 0;
+            ps[i] = 0;
+        }
         fp = std::fopen("gap_sequence.csv", "w");
         if (fp) {
             std::fprintf(fp, "index,gap,p\n");
             if (rc == VOSS_OK)
                 for (int i = 0; i < VOSS_GAP_SEQ_SIZE; i++)
                     if (gs[i] > 0) std::fprintf(fp, "%d,%d,%llu\n", i, gs[i], (unsigned long long)ps[i]);
+            (void)rows;
             std::fclose(fp);
             std::printf("OK gap_sequence.csv\n");
         }

@@ -123,7 +123,8 @@ class Tuning(ctypes.Structure):
 
 EXPORTS = [
     "voss_create", "voss_destroy", "voss_last_error", "voss_device_name",
-    "voss_set_tuning", "voss_get_tuning", "voss_run_plain", "voss_sample_plain",
+    "voss_set_tuning", "voss_get_tuning", "voss_run_plain", "voss_run_plain_range",
+    "voss_sample_plain",
     "voss_run_interval", "voss_run_batch", "voss_run_batch_plain", "voss_mr_batch",
     "voss_mr_batch_device", "voss_verify_gaps", "voss_device_alloc", "voss_device_free",
     "voss_memcpy_h2d", "voss_memcpy_d2h",
@@ -151,6 +152,7 @@ def lib():
         L.voss_get_tuning.argtypes = [vp, P(Tuning)]
         L.voss_run_plain.argtypes = [vp, u64, u64, P(Result)]
         L.voss_sample_plain.argtypes = [vp, u64, u64, vp, vp, P(u64)]
+        L.voss_run_plain_range.argtypes = [vp, u64, u64, u64, u64, P(Result)]
         L.voss_run_interval.argtypes = [vp, u64, u64, P(Result)]
         L.voss_run_batch.argtypes = [vp, vp, vp, sz, vp, P(Result)]
         L.voss_run_batch_plain.argtypes = [vp, vp, vp, sz, vp, P(Result)]
@@ -267,6 +269,21 @@ class Engine:
         r = self._result(large_cap)
         _check(lib().voss_run_plain(self._h, N, seg_num, ctypes.byref(r)))
         return result_to_dict(r)
+
+    def run_plain_range(self, N, seg_num, seg_begin, seg_end, large_cap=1 << 20):
+        r = self._result(large_cap)
+        _check(lib().voss_run_plain_range(self._h, N, seg_num, seg_begin, seg_end,
+                                          ctypes.byref(r)))
+        return result_to_dict(r)
+
+    def sample_plain(self, N, seg_num):
+        """gap_sequence sample: (gap_seq int32[5e6], pos_seq uint64[5e6], rows)."""
+        gs = np.zeros(GAP_SEQ_SIZE, dtype=np.int32)
+        ps = np.zeros(GAP_SEQ_SIZE, dtype=np.uint64)
+        rows = ctypes.c_uint64()
+        _check(lib().voss_sample_plain(self._h, N, seg_num, gs.ctypes.data, ps.ctypes.data,
+                                       ctypes.byref(rows)))
+        return gs, ps, int(rows.value)
 
     def run_interval(self, lo, hi, large_cap=1 << 16):
         r = self._result(large_cap)
