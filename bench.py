@@ -74,19 +74,53 @@ def cpu_baseline(threads):
                       f"{threads} threads"}
 
 
-def mr_candidates(eng, count, seed=0x564F5353 + 2):
-    """Half odd uniform in [2^40, 2^64), half gap endpoints of a sieve run."""
-    rng = np.random.default_rng(seed)
-    half = count // 2
-    rnd = rng.integers(2**40, 2**64 - 1, size=count - half, dtype=np.uint64,
-                       endpoint=False) | np.uint64(1)
-    r = eng.run_interval(10**15, 10**15 + 2 * 10**10, large_cap=1 << 16)
-    pos = r["large"]["position"].astype(np.uint64)
-    ends = np.concatenate([pos, pos - r["large"]["gap"].astype(np.uint64)])
-    pick = ends[rng.integers(0, len(ends), size=half)]
-    c = np.concatenate([rnd, pick])
-    rng.shuffle(c)
-    return c
+def batch_leg(eng, W, with_plain=True):
+    """Config 4: 4096 seeded intervals (starts in [1e12, 1e16), lengths
+    log-uniform in [1e4, 1e9]) in one persistent launch sequence, against
+    one launch sequence per interval; outputs compared bit for bit."""
+    iv = W.batch_intervals(4096)
+    lo = [a for a, _ in iv]
+    hi = [b for _, b in iv]
+    numbers = sum(b - a for a, b in iv)
+    eng.run_interval(10**16, 10**16 + 10**6)  # base primes to 1e8: outside the timing
+    t = time.perf_counter()
+    rb, lbuf, st = eng.run_batch(lo, hi, large_cap=64, raw=True)
+    t_batch = time.perf_counter() - t
+    out = {"intervals": len(iv), "numbers": numbers, "batched_s": t_batch,
+           "batched_numbers_per_s": numbers / t_batch, "sieve_ms": st["sieve_ms"],
+           "bucket_ms": st["bucket_ms"], "windows": st["windows"]}
+    if with_plain:
+        t = time.perf_counter()
+        rp, lbuf2, sp = eng.run_batch_plain(lo, hi, large_cap=64, raw=True)
+        t_plain = time.perf_counter() - t
+        import vossgpu
+        off = vossgpu.IntervalResult.large.offset  # compare all fields but the pointer
+
+        def lg(r):
+            return vossgpu.large_array(r.large, min(r.large_count, r.large_capacity)).tobytes()
+        same = all(bytes(a)[:off] == bytes(b)[:off] and lg(a) == lg(b) for a, b in zip(rb, rp))
+        out.update({"plain_s": t_plain, "batched_over_plain": t_plain / t_batch,
+                    "bit_exact": bool(same)})
+    # gap endpoints for the Miller-Rabin leg
+    ends = []
+    for k in range(len(iv)):
+        for i in range(min(rb[k].large_count, rb[k].large_capacity)):
+            g = rb[k].large[i]
+            ends += [g.position, g.position - g.gap]
+    return out, np.array(ends, dtype=np.uint64)
+
+
+def interval_leg(eng):
+    """Config 3: [1e13, 1e13 + 1e12), base primes to 3.32e6."""
+    lo, hi = 10**13, 10**13 + 10**12
+    eng.run_interval(lo, lo + 10**9)
+    t = time.perf_counter()
+    r = eng.run_interval(lo, hi, large_cap=1 << 22)
+    dt = time.perf_counter() - t
+    assert r["total_primes"] == 33353349498, r["total_primes"]
+    return {"lo": lo, "hi": hi, "seconds": dt, "numbers_per_s": (hi - lo) / dt,
+            "pi": r["total_primes"], "sieve_ms": r["sieve_ms"], "bucket_ms": r["bucket_ms"],
+            "large_gaps": r["large_count"]}
 
 
 def main():
@@ -96,6 +130,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mr-count", type=int, default=10**7)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the 1e11 steps and MR (for profiling runs)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the per-interval comparison form of the batch leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -123,10 +161,31 @@ def main():
     avg_sieve_s = statistics.mean(sieve_ms) / 1e3
     achieved = lds_bytes / avg_sieve_s / 1e9
 
-    # Miller-Rabin tests/s on device-resident candidates
-    cand = mr_candidates(eng, args.mr_count)
+    import workloads as W
+    legs = {}
+    if not args.headline_only:
+        legs["interval_1e13"] = interval_leg(eng)
+        legs["batch_4096"], ends = batch_leg(eng, W, with_plain=not args.no_plain)
+    else:
+        ends = np.array([10**12 + 39], dtype=np.uint64)
+
+    # Miller-Rabin tests/s on device-resident candidates (config 4's set)
+    cand = W.mr_candidates(args.mr_count, ends)
     verdict, mr_ms = eng.mr_device(cand, repeats=3)
     mr_rate = len(cand) / (statistics.median(mr_ms) / 1e3)
+    mr = {"tests_per_s": mr_rate, "count": int(len(cand)), "primes": int(verdict.sum()),
+          "kernel_ms": mr_ms}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        sample = cand[: min(len(cand), 2 * 10**6)]
+        t = time.perf_counter()
+        ref = O.is_prime_batch(sample, threads=threads)
+        dt = time.perf_counter() - t
+        assert np.array_equal(ref, verdict[: len(sample)])
+        mr["cpu_baseline"] = {"value": len(sample) / dt, "unit": "tests/s", "cores": threads,
+                              "kind": "port", "sample": f"first {len(sample)} candidates, "
+                              "is_prime_u64 restated with __uint128_t (oracle, OpenMP)"}
 
     out = {
         "metric": "sieved numbers/s at N=1e11 (plain [1,N], SEG_NUM=3e10, bit-exact outputs)",
@@ -140,8 +199,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32/u64 integer",
-        "data": "synthetic (the range [1, 1e11] itself; MR candidates seeded SplitMix-free "
-                "numpy PCG64 draws + sieve gap endpoints)",
+        "data": "synthetic: the range [1, 1e11] itself; batch intervals and MR candidates "
+                "from seeded SplitMix64 streams (seed 0x564F5353+k)",
         "config": {"workload": "plain sieve+gaps [1, 1e11], SEG_NUM=3e10 (4 logical segments), "
                                "131072-byte LDS windows", "N": N_BENCH, "SEG_NUM": SEG_NUM,
                    "windows": r["windows"], "device": eng.name},
@@ -151,8 +210,8 @@ def main():
                      "algorithmic_bytes": lds_bytes, "m_eff": m_eff,
                      "avg_kernel_ms": statistics.mean(sieve_ms)},
         "device_ms_per_step": statistics.mean(dev_ms),
-        "mr": {"tests_per_s": mr_rate, "count": int(len(cand)),
-               "primes": int(verdict.sum()), "kernel_ms": mr_ms},
+        "mr": mr,
+        "legs": legs,
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -248,3 +248,20 @@ def test_plain_range_pieces_compose(engine):
     assert np.array_equal(modq, whole["modq"])
     assert (total, last, c1, c3, nlarge) == (whole["total_primes"], whole["last_prime"],
                                              whole["c1"], whole["c3"], whole["large_count"])
+
+
+def test_large_gap_truncation_is_canonical(engine, oracle):
+    """A caller capacity smaller than the number of large gaps keeps the
+    first K in canonical order (merit desc, position asc) in every form."""
+    lo, hi = 10**12, 10**12 + 2 * 10**9
+    print("[gpu] truncation K=8", flush=True)
+    ref = oracle.run_interval(lo, hi)
+    assert ref["large_count"] > 100
+    top = [(p, g, float(np.float32(m))) for p, g, m in ref["large"][:8]]
+    single = engine.run_interval(lo, hi, large_cap=8)
+    assert single["large"].tolist() == top and single["large_count"] == ref["large_count"]
+    lo2, hi2 = [lo, 10**13], [hi, 10**13 + 10**9]
+    (b0, b1), _ = engine.run_batch(lo2, hi2, large_cap=8)
+    (p0, p1), _ = engine.run_batch_plain(lo2, hi2, large_cap=8)
+    assert b0["large"].tolist() == top == p0["large"].tolist()
+    assert b1["large"].tolist() == p1["large"].tolist()

@@ -626,7 +626,11 @@ int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
     if (P.njobs == 0) return VOSS_OK;
     HIPCHK(c->d_acc.reserve(P.njobs));
     HIPCHK(c->d_large_n.reserve(1));
-    const uint64_t lcap = std::max<uint64_t>(large_cap_total, 1);
+    // The device keeps every large-gap candidate of a round, so that the
+    // caller's top-K (canonical order) does not depend on atomic order; a
+    // round that overflows the list is re-run with a larger one.
+    (void)large_cap_total;
+    uint64_t lcap = std::max<uint64_t>(c->d_large.cap, 1u << 20);
     HIPCHK(c->d_large.reserve(lcap));
     const uint32_t block_per_cu = std::max<uint32_t>(c->tune.wg_per_cu, 1);
     const uint32_t grid = (uint32_t)c->num_cus * block_per_cu;
@@ -642,7 +646,8 @@ int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
     std::vector<Round> rounds;
     make_rounds(c, P, rounds);
     ExecTimes tmb;
-    for (const Round& R : rounds) {
+    for (size_t ri = 0; ri < rounds.size(); ri++) {
+        const Round& R = rounds[ri];
         const size_t off = R.off;
         const uint32_t nd = (uint32_t)R.nd;
         int brc = run_buckets(c, R, &tmb);
@@ -676,7 +681,13 @@ int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
         sieve_ms += sms;
         const JobAcc* acc = c->h_acc.p;
         const SegRec* rec = c->h_rec.p;
-        const uint64_t nkeep = std::min<uint64_t>(c->h_large_n.p[0], lcap);
+        if (c->h_large_n.p[0] > lcap) {
+            lcap = c->h_large_n.p[0] + (c->h_large_n.p[0] >> 2);
+            HIPCHK(c->d_large.reserve(lcap));
+            ri--;  // same round again with room for every candidate
+            continue;
+        }
+        const uint64_t nkeep = c->h_large_n.p[0];
         HIPCHK(c->h_large.reserve(nkeep));
         if (nkeep) {
             HIPCHK(hipMemcpyAsync(c->h_large.p, c->d_large.p, nkeep * sizeof(LargeRaw),
