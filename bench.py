@@ -110,6 +110,77 @@ def batch_leg(eng, W, with_plain=True):
     return out, np.array(ends, dtype=np.uint64)
 
 
+def serve_leg(eng, W, n_req=300, n_cal=80):
+    """Config 5: a Poisson stream (half interval requests with start in
+    [1e10, 1e16) and length log-uniform in [1e5, 1e10], half MR batches of
+    1e3..1e6 candidates) at 0.7x the capacity of the plain path, which runs
+    one request at a time in arrival order.  Latency = completion - arrival."""
+    import threading
+    reqs = W.serve_requests(n_req + n_cal, 1.0)
+    rng = np.random.default_rng(W.SEED0 + 4)
+    payload = {}
+    for k, (_, kind, p) in enumerate(reqs):
+        if kind == "mr":
+            payload[k] = rng.integers(2**40, 2**63, size=p, dtype=np.uint64) | np.uint64(1)
+
+    def run_alone(k):
+        _, kind, p = reqs[k]
+        if kind == "interval":
+            return eng.run_batch([p[0]], [p[1]], large_cap=64)[0][0]["total_primes"]
+        return int(eng.mr(payload[k]).sum())
+
+    eng.run_interval(10**16, 10**16 + 10**6)  # base primes to 1e8 outside the timing
+    t = time.perf_counter()
+    for k in range(n_req, n_req + n_cal):  # calibration requests (not in the stream)
+        run_alone(k)
+    cap = n_cal / (time.perf_counter() - t)
+    lam = 0.7 * cap
+    arrivals = [a / lam for a, _, _ in reqs[:n_req]]  # the stream at rate lam
+
+    # plain path: one request at a time, FIFO
+    lat_plain = []
+    t0 = time.perf_counter()
+    for k in range(n_req):
+        now = time.perf_counter() - t0
+        if now < arrivals[k]:
+            time.sleep(arrivals[k] - now)
+        run_alone(k)
+        lat_plain.append(time.perf_counter() - t0 - arrivals[k])
+    span_plain = time.perf_counter() - t0
+
+    # continuous batching
+    ids = [None] * n_req
+    with eng.serve(max_active=64, windows_per_step=2048, large_capacity=64) as srv:
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k in range(n_req):
+                now = time.perf_counter() - t0
+                if now < arrivals[k]:
+                    time.sleep(arrivals[k] - now)
+                _, kind, p = reqs[k]
+                ids[k] = srv.submit_interval(*p) if kind == "interval" else srv.submit_mr(payload[k])
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        lat_srv = [srv.wait(rid)[1] for rid in ids]
+        span_srv = time.perf_counter() - t0
+
+    numbers = sum(p[1] - p[0] for _, kind, p in reqs[:n_req] if kind == "interval")
+    tests = sum(p for _, kind, p in reqs[:n_req] if kind == "mr")
+
+    def pct(x, q):
+        return float(np.percentile(np.array(x) * 1e3, q))
+
+    return {"requests": n_req, "rate_req_per_s": lam, "plain_capacity_req_per_s": cap,
+            "interval_numbers": numbers, "mr_tests": tests,
+            "served": {"p50_ms": pct(lat_srv, 50), "p99_ms": pct(lat_srv, 99),
+                       "makespan_s": span_srv, "req_per_s": n_req / span_srv},
+            "plain": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
+                      "makespan_s": span_plain, "req_per_s": n_req / span_plain}}
+
+
 def interval_leg(eng):
     """Config 3: [1e13, 1e13 + 1e12), base primes to 3.32e6."""
     lo, hi = 10**13, 10**13 + 10**12
@@ -166,6 +237,7 @@ def main():
     if not args.headline_only:
         legs["interval_1e13"] = interval_leg(eng)
         legs["batch_4096"], ends = batch_leg(eng, W, with_plain=not args.no_plain)
+        legs["serve"] = serve_leg(eng, W)
     else:
         ends = np.array([10**12 + 39], dtype=np.uint64)
 

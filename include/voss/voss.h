@@ -194,6 +194,40 @@ int voss_mr_batch_device(voss_ctx* ctx, const uint64_t* d_n, size_t count,
 int voss_verify_gaps(voss_ctx* ctx, const uint64_t* p_after,
                      const int32_t* gap, size_t count, uint8_t* ok);
 
+/* ---------------------------------------------------------------- serving */
+
+/* Continuous-batching server (no reference counterpart: the reference runs
+ * one range per process).  A scheduler thread owns the context while the
+ * server runs.  Every step it admits queued requests while request slots
+ * and the HBM budget allow, then runs one sieve round made of up to
+ * windows_per_step LDS windows taken round-robin from all active interval
+ * requests (their accumulators stay in HBM slots across steps), and one
+ * Miller-Rabin launch over the admitted verify batches on a second stream,
+ * overlapping the sieve.  Results are exactly those of voss_run_batch /
+ * voss_mr_batch on the same request. */
+typedef struct voss_server voss_server;
+
+typedef struct {
+    uint32_t max_active;       /* interval requests resident at once (default 64) */
+    uint32_t windows_per_step; /* LDS windows per sieve round (default 4096) */
+    uint64_t hbm_budget_bytes; /* admission budget for per-request device state */
+    uint32_t large_capacity;   /* large gaps kept per interval request (default 64) */
+    uint32_t mr_per_step;      /* candidates per Miller-Rabin launch (default 2^22) */
+} voss_serve_config;
+
+int voss_serve_start(voss_ctx* ctx, const voss_serve_config* cfg, voss_server** out);
+int voss_serve_submit_interval(voss_server* s, uint64_t lo, uint64_t hi, uint64_t* id);
+int voss_serve_submit_mr(voss_server* s, const uint64_t* n, size_t count, uint64_t* id);
+/* *done = 1 when request id has finished; then out (interval requests, its
+ * large buffer/capacity honoured) or verdict (MR requests, count bytes)
+ * receive the result and latency_s the time from submit to completion. */
+int voss_serve_poll(voss_server* s, uint64_t id, int* done, voss_interval_result* out,
+                    uint8_t* verdict, double* latency_s);
+int voss_serve_wait(voss_server* s, uint64_t id, voss_interval_result* out,
+                    uint8_t* verdict, double* latency_s);
+/* Finishes every submitted request, stops the scheduler, frees the server. */
+int voss_serve_stop(voss_server* s);
+
 /* --------------------------------------------------- device memory helpers */
 
 int voss_device_alloc(voss_ctx* ctx, size_t bytes, void** dptr);

@@ -1353,3 +1353,434 @@ int voss_memcpy_d2h(voss_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ serving
+
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+struct ServeReq {
+    uint64_t id = 0;
+    bool is_mr = false;
+    uint64_t lo = 0, hi = 0;
+    std::vector<uint64_t> cand;
+    std::vector<uint8_t> verdict;
+    double t_submit = 0, t_done = 0;
+    uint64_t hbm = 0;
+    // interval requests
+    std::vector<SegDesc> win;  // planned windows, job = slot
+    size_t next = 0;
+    int slot = -1;
+    JobHost J;
+    uint64_t carry_last = 0;
+    bool carry_has = false;
+    voss_interval_result res{};
+    std::vector<voss_large_gap> large;
+    int status = VOSS_OK;
+    bool done = false;
+};
+
+using ReqPtr = std::shared_ptr<ServeReq>;
+
+}  // namespace
+
+struct voss_server {
+    voss_ctx* c = nullptr;
+    voss_serve_config cfg{};
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ReqPtr> queue;
+    std::unordered_map<uint64_t, ReqPtr> all;
+    uint64_t next_id = 1;
+    bool stopping = false;
+    int err = VOSS_OK;
+    std::string errmsg;
+    // owned by the scheduler thread
+    std::vector<ReqPtr> active;
+    std::vector<int> free_slots;
+    uint64_t hbm_used = 0;
+    DevBuf<JobAcc> d_acc;
+    HostBuf<JobAcc> h_acc;
+    hipStream_t mr_stream = nullptr;
+    DevBuf<uint64_t> d_mr_n;
+    DevBuf<uint8_t> d_mr_v;
+    DevBuf<uint32_t> d_surv;
+    DevBuf<uint32_t> d_nsurv;
+};
+
+namespace {
+
+constexpr uint64_t kSlotBytes = sizeof(JobAcc);
+
+uint64_t req_hbm(const voss_server* s, const ServeReq& r) {
+    if (r.is_mr) return r.cand.size() * (8 + 1 + 4);
+    return kSlotBytes + (uint64_t)s->cfg.large_capacity * sizeof(LargeRaw);
+}
+
+// One sieve step over windows of several resident requests: buckets, sieve
+// and stitch per round, accumulators left in the requests' HBM slots; the
+// pairs across round/step edges and the large-gap candidates go to the
+// requests on the host.
+int serve_sieve_step(voss_server* s, const std::vector<std::pair<ReqPtr, size_t>>& take) {
+    voss_ctx* c = s->c;
+    Plan P;
+    P.njobs = s->cfg.max_active;
+    std::vector<ServeReq*> by_slot(s->cfg.max_active, nullptr);
+    for (const auto& t : take) {
+        ServeReq* r = t.first.get();
+        by_slot[r->slot] = r;
+        for (size_t k = 0; k < t.second; k++) P.desc.push_back(r->win[r->next + k]);
+    }
+    if (P.desc.empty()) return VOSS_OK;
+    std::vector<Round> rounds;
+    make_rounds(c, P, rounds);
+    uint64_t lcap = std::max<uint64_t>(c->d_large.cap, 1u << 20);
+    HIPCHK(c->d_large.reserve(lcap));
+    HIPCHK(c->d_large_n.reserve(1));
+    HIPCHK(c->h_large_n.reserve(1));
+    const uint32_t grid = (uint32_t)c->num_cus * std::max<uint32_t>(c->tune.wg_per_cu, 1);
+    ExecTimes tmb;
+    for (size_t ri = 0; ri < rounds.size(); ri++) {
+        const Round& R = rounds[ri];
+        const uint32_t nd = (uint32_t)R.nd;
+        int rc = run_buckets(c, R, &tmb);
+        if (rc) return rc;
+        HIPCHK(c->d_desc.reserve(nd));
+        HIPCHK(c->d_rec.reserve(nd));
+        HIPCHK(c->h_desc.reserve(nd));
+        HIPCHK(c->h_rec.reserve(nd));
+        std::memcpy(c->h_desc.p, R.desc.data(), nd * sizeof(SegDesc));
+        HIPCHK(hipMemcpyAsync(c->d_desc.p, c->h_desc.p, nd * sizeof(SegDesc),
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemsetAsync(c->d_large_n.p, 0, sizeof(unsigned long long), c->stream));
+        SieveArgs a = make_args(c, nd, lcap);
+        a.acc = s->d_acc.p;  // resident per-request slots
+        HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, c->stream));
+        HIPCHK(launch_stitch(a, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_rec.p, c->d_rec.p, nd * sizeof(SegRec), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_large_n.p, c->d_large_n.p, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->h_large_n.p[0] > lcap) {
+            // the slots already hold this round's sums: this cannot be re-run
+            return fail(VOSS_ENOMEM, "large-gap list overflow in a serving step");
+        }
+        const uint64_t nl = c->h_large_n.p[0];
+        HIPCHK(c->h_large.reserve(nl));
+        if (nl) {
+            HIPCHK(hipMemcpy(c->h_large.p, c->d_large.p, nl * sizeof(LargeRaw),
+                             hipMemcpyDeviceToHost));
+        }
+        for (uint64_t k = 0; k < nl; k++) {
+            const LargeRaw& x = c->h_large.p[k];
+            ServeReq* r = by_slot[x.job];
+            r->J.large.push_back({x.p_before + x.gap, (int32_t)x.gap, merit_of(x.p_before, x.gap)});
+        }
+        // edges between this round and what the request saw before
+        std::vector<uint8_t> seen(s->cfg.max_active, 0);
+        for (uint32_t i = 0; i < nd; i++) {
+            const SegDesc& d = R.desc[i];
+            const SegRec& rc2 = c->h_rec.p[i];
+            if (rc2.count == 0) continue;
+            ServeReq* r = by_slot[d.job];
+            if (!seen[d.job]) {
+                seen[d.job] = 1;
+                if (r->carry_has) host_pair(r->J, r->carry_last, rc2.first, true);
+            }
+            r->carry_has = true;
+            r->carry_last = rc2.last;
+        }
+    }
+    return VOSS_OK;
+}
+
+void serve_fail_all(voss_server* s, int code, const std::string& msg) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->err = code;
+    s->errmsg = msg;
+    for (auto& kv : s->all)
+        if (!kv.second->done) {
+            kv.second->status = code;
+            kv.second->done = true;
+        }
+    s->cv_done.notify_all();
+}
+
+void serve_loop(voss_server* s) {
+    voss_ctx* c = s->c;
+    if (hipSetDevice(c->device) != hipSuccess) {
+        serve_fail_all(s, VOSS_EDEVICE, "hipSetDevice");
+        return;
+    }
+    const uint32_t W = c->tune.window_bytes;
+    while (true) {
+        std::vector<ReqPtr> admitted_iv, admitted_mr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stopping || !s->queue.empty() || !s->active.empty(); });
+            if (s->queue.empty() && s->active.empty() && s->stopping) break;
+            // admission by slots and HBM budget, in arrival order
+            uint64_t mr_take = 0;
+            while (!s->queue.empty()) {
+                ReqPtr r = s->queue.front();
+                const uint64_t cost = req_hbm(s, *r);
+                const bool fits = s->hbm_used + cost <= s->cfg.hbm_budget_bytes ||
+                                  (s->active.empty() && admitted_iv.empty() && admitted_mr.empty());
+                if (!fits) break;
+                if (r->is_mr) {
+                    if (!admitted_mr.empty() && mr_take + r->cand.size() > s->cfg.mr_per_step) break;
+                    mr_take += r->cand.size();
+                    admitted_mr.push_back(r);
+                } else {
+                    if (s->free_slots.empty()) break;
+                    r->slot = s->free_slots.back();
+                    s->free_slots.pop_back();
+                    admitted_iv.push_back(r);
+                    s->active.push_back(r);
+                }
+                r->hbm = cost;
+                s->hbm_used += cost;
+                s->queue.pop_front();
+            }
+        }
+        int rc = VOSS_OK;
+        // plan the admitted interval requests; fresh accumulator slots
+        for (auto& r : admitted_iv) {
+            if (r->hi > 7) {
+                if ((rc = ensure_primes(c, r->hi / 30 * 30 + 30ull * W + 30))) break;
+                Plan P1;
+                P1.njobs = 1;
+                const uint64_t a = std::max<uint64_t>(r->lo, 7), b = r->hi - 1;
+                plan_range(c, P1, a, b, (uint32_t)r->slot, 0, W, (a / 30) & ~4095ull);
+                r->win.swap(P1.desc);
+            }
+            if (hipSuccess != launch_init_acc(s->d_acc.p + r->slot, 1, c->stream)) {
+                rc = fail(VOSS_EDEVICE, "init_acc");
+                break;
+            }
+        }
+        // Miller-Rabin batch on the second stream (overlaps the sieve step)
+        uint64_t mr_n = 0;
+        std::vector<uint64_t> all_n;  // source of the async copy: lives past the step
+        if (rc == VOSS_OK && !admitted_mr.empty()) {
+            for (auto& r : admitted_mr) mr_n += r->cand.size();
+            all_n.reserve(mr_n);
+            for (auto& r : admitted_mr) all_n.insert(all_n.end(), r->cand.begin(), r->cand.end());
+            if (s->d_mr_n.reserve(mr_n) || s->d_mr_v.reserve(mr_n) || s->d_surv.reserve(mr_n) ||
+                s->d_nsurv.reserve(1))
+                rc = fail(VOSS_ENOMEM, "MR buffers");
+            else if (hipMemcpyAsync(s->d_mr_n.p, all_n.data(), mr_n * 8, hipMemcpyHostToDevice,
+                                    s->mr_stream) != hipSuccess ||
+                     launch_mr(s->d_mr_n.p, mr_n, s->d_mr_v.p, s->d_surv.p, s->d_nsurv.p,
+                               c->num_cus, s->mr_stream) != hipSuccess)
+                rc = fail(VOSS_EDEVICE, "MR launch");
+        }
+        // sieve step: windows round-robin over the resident requests
+        std::vector<std::pair<ReqPtr, size_t>> take;
+        if (rc == VOSS_OK && !s->active.empty()) {
+            size_t budget = s->cfg.windows_per_step;
+            const size_t n_act = s->active.size();
+            const size_t quota = std::max<size_t>(1, budget / n_act);
+            for (auto& r : s->active) {
+                const size_t left = r->win.size() - r->next;
+                const size_t k = std::min(left, quota);
+                take.push_back({r, k});
+            }
+            rc = serve_sieve_step(s, take);
+            for (auto& t : take) t.first->next += t.second;
+        }
+        if (mr_n && hipStreamSynchronize(s->mr_stream) != hipSuccess && rc == VOSS_OK)
+            rc = fail(VOSS_EDEVICE, "MR sync");
+        if (rc != VOSS_OK) {
+            serve_fail_all(s, rc, voss_last_error());
+            return;
+        }
+        // completions
+        std::vector<ReqPtr> finished;
+        for (size_t i = 0; i < s->active.size();) {
+            ReqPtr r = s->active[i];
+            if (r->next < r->win.size()) { i++; continue; }
+            if (s->h_acc.reserve(1) ||
+                hipMemcpy(s->h_acc.p, s->d_acc.p + r->slot, sizeof(JobAcc), hipMemcpyDeviceToHost)) {
+                serve_fail_all(s, VOSS_EDEVICE, "reading a request's accumulators");
+                return;
+            }
+            if (!r->win.empty()) add_acc(r->J, *s->h_acc.p);
+            r->large.resize(s->cfg.large_capacity);
+            r->res.large = r->large.data();
+            r->res.large_capacity = s->cfg.large_capacity;
+            finish_interval(r->J, r->lo, r->hi, &r->res);
+            r->large.resize(std::min<uint64_t>(r->res.large_count, s->cfg.large_capacity));
+            if (r->J.overflow) r->status = VOSS_ERANGE;
+            finished.push_back(r);
+            s->active.erase(s->active.begin() + (long)i);
+        }
+        for (auto& r : admitted_mr) {
+            r->verdict.resize(r->cand.size());
+            finished.push_back(r);
+        }
+        if (mr_n) {
+            std::vector<uint8_t> v(mr_n);
+            if (hipMemcpy(v.data(), s->d_mr_v.p, mr_n, hipMemcpyDeviceToHost) != hipSuccess) {
+                serve_fail_all(s, VOSS_EDEVICE, "MR verdicts");
+                return;
+            }
+            size_t o = 0;
+            for (auto& r : admitted_mr) {
+                std::memcpy(r->verdict.data(), v.data() + o, r->cand.size());
+                o += r->cand.size();
+            }
+        }
+        if (!finished.empty()) {
+            const double t = now_s();
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (auto& r : finished) {
+                r->t_done = t;
+                r->done = true;
+                s->hbm_used -= r->hbm;
+                if (!r->is_mr) s->free_slots.push_back(r->slot);
+            }
+            s->cv_done.notify_all();
+        }
+    }
+}
+
+int serve_result(voss_server* s, ReqPtr r, voss_interval_result* out, uint8_t* verdict,
+                 double* latency_s) {
+    if (latency_s) *latency_s = r->t_done - r->t_submit;
+    if (r->status != VOSS_OK) return fail(r->status, s->errmsg.empty() ? "request failed" : s->errmsg);
+    if (r->is_mr) {
+        if (verdict && !r->verdict.empty()) std::memcpy(verdict, r->verdict.data(), r->verdict.size());
+    } else if (out) {
+        voss_large_gap* lg = out->large;
+        const uint64_t cap = out->large_capacity;
+        *out = r->res;
+        out->large = lg;
+        out->large_capacity = cap;
+        if (lg && cap) {
+            const size_t n = std::min<size_t>(r->large.size(), cap);
+            std::memcpy(lg, r->large.data(), n * sizeof(voss_large_gap));
+        }
+    }
+    s->all.erase(r->id);
+    return VOSS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int voss_serve_start(voss_ctx* c, const voss_serve_config* cfg, voss_server** out) {
+    if (!c || !out) return fail(VOSS_EINVAL, "NULL argument");
+    *out = nullptr;
+    int rc = set_device(c);
+    if (rc) return rc;
+    voss_server* s = new voss_server();
+    s->c = c;
+    voss_serve_config d{64, 4096, 8ull << 30, 64, 1u << 22};
+    if (cfg) {
+        if (cfg->max_active) d.max_active = cfg->max_active;
+        if (cfg->windows_per_step) d.windows_per_step = cfg->windows_per_step;
+        if (cfg->hbm_budget_bytes) d.hbm_budget_bytes = cfg->hbm_budget_bytes;
+        if (cfg->large_capacity) d.large_capacity = cfg->large_capacity;
+        if (cfg->mr_per_step) d.mr_per_step = cfg->mr_per_step;
+    }
+    s->cfg = d;
+    hipError_t e = s->d_acc.reserve(d.max_active);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->mr_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return hip_fail(e, "serve setup");
+    }
+    for (int i = (int)d.max_active - 1; i >= 0; i--) s->free_slots.push_back(i);
+    s->th = std::thread(serve_loop, s);
+    *out = s;
+    return VOSS_OK;
+}
+
+int voss_serve_submit_interval(voss_server* s, uint64_t lo, uint64_t hi, uint64_t* id) {
+    if (!s || !id) return fail(VOSS_EINVAL, "NULL argument");
+    if (hi < lo || hi > (1ull << 62)) return fail(VOSS_EINVAL, "bad interval");
+    auto r = std::make_shared<ServeReq>();
+    r->lo = lo;
+    r->hi = hi;
+    r->t_submit = now_s();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->err) return fail(s->err, s->errmsg);
+    if (s->stopping) return fail(VOSS_EBUSY, "server is stopping");
+    r->id = s->next_id++;
+    s->queue.push_back(r);
+    s->all[r->id] = r;
+    *id = r->id;
+    s->cv_work.notify_one();
+    return VOSS_OK;
+}
+
+int voss_serve_submit_mr(voss_server* s, const uint64_t* n, size_t count, uint64_t* id) {
+    if (!s || !id || (count && !n)) return fail(VOSS_EINVAL, "NULL argument");
+    if (count >= 0xFFFFFFFFull) return fail(VOSS_EINVAL, "count must be below 2^32");
+    auto r = std::make_shared<ServeReq>();
+    r->is_mr = true;
+    r->cand.assign(n, n + count);
+    r->t_submit = now_s();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->err) return fail(s->err, s->errmsg);
+    if (s->stopping) return fail(VOSS_EBUSY, "server is stopping");
+    r->id = s->next_id++;
+    s->queue.push_back(r);
+    s->all[r->id] = r;
+    *id = r->id;
+    s->cv_work.notify_one();
+    return VOSS_OK;
+}
+
+int voss_serve_poll(voss_server* s, uint64_t id, int* done, voss_interval_result* out,
+                    uint8_t* verdict, double* latency_s) {
+    if (!s || !done) return fail(VOSS_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return fail(VOSS_ENOTFOUND, "unknown request id");
+    *done = it->second->done ? 1 : 0;
+    if (!*done) return VOSS_OK;
+    return serve_result(s, it->second, out, verdict, latency_s);
+}
+
+int voss_serve_wait(voss_server* s, uint64_t id, voss_interval_result* out, uint8_t* verdict,
+                    double* latency_s) {
+    if (!s) return fail(VOSS_EINVAL, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return fail(VOSS_ENOTFOUND, "unknown request id");
+    ReqPtr r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    return serve_result(s, r, out, verdict, latency_s);
+}
+
+int voss_serve_stop(voss_server* s) {
+    if (!s) return fail(VOSS_EINVAL, "NULL argument");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+        s->cv_work.notify_all();
+    }
+    if (s->th.joinable()) s->th.join();
+    const int err = s->err;
+    (void)hipSetDevice(s->c->device);
+    if (s->mr_stream) (void)hipStreamDestroy(s->mr_stream);
+    s->d_acc.release();
+    s->h_acc.release();
+    s->d_mr_n.release();
+    s->d_mr_v.release();
+    s->d_surv.release();
+    s->d_nsurv.release();
+    delete s;
+    return err;
+}
+
+}  // extern "C"

@@ -115,6 +115,12 @@ def interval_to_dict(r):
     }
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("max_active", ctypes.c_uint32), ("windows_per_step", ctypes.c_uint32),
+                ("hbm_budget_bytes", ctypes.c_uint64), ("large_capacity", ctypes.c_uint32),
+                ("mr_per_step", ctypes.c_uint32)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("window_bytes", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
                 ("tier_a_limit", ctypes.c_uint32), ("bucket_limit", ctypes.c_uint32),
@@ -127,7 +133,8 @@ EXPORTS = [
     "voss_sample_plain",
     "voss_run_interval", "voss_run_batch", "voss_run_batch_plain", "voss_mr_batch",
     "voss_mr_batch_device", "voss_verify_gaps", "voss_device_alloc", "voss_device_free",
-    "voss_memcpy_h2d", "voss_memcpy_d2h",
+    "voss_memcpy_h2d", "voss_memcpy_d2h", "voss_serve_start", "voss_serve_submit_interval",
+    "voss_serve_submit_mr", "voss_serve_poll", "voss_serve_wait", "voss_serve_stop",
 ]
 
 _lib = None
@@ -163,6 +170,12 @@ def lib():
         L.voss_device_free.argtypes = [vp, vp]
         L.voss_memcpy_h2d.argtypes = [vp, vp, vp, sz]
         L.voss_memcpy_d2h.argtypes = [vp, vp, vp, sz]
+        L.voss_serve_start.argtypes = [vp, P(ServeConfig), P(vp)]
+        L.voss_serve_submit_interval.argtypes = [vp, u64, u64, P(u64)]
+        L.voss_serve_submit_mr.argtypes = [vp, vp, sz, P(u64)]
+        L.voss_serve_poll.argtypes = [vp, u64, P(i), vp, vp, P(ctypes.c_double)]
+        L.voss_serve_wait.argtypes = [vp, u64, vp, vp, P(ctypes.c_double)]
+        L.voss_serve_stop.argtypes = [vp]
         for name in EXPORTS:
             if name not in ("voss_destroy", "voss_last_error"):
                 getattr(L, name).restype = ctypes.c_int
@@ -347,6 +360,13 @@ class Engine:
             lib().voss_device_free(self._h, dv)
         return out, times
 
+    def serve(self, max_active=0, windows_per_step=0, hbm_budget_bytes=0, large_capacity=0,
+              mr_per_step=0):
+        """A continuous-batching Server on this engine (which it owns until
+        the server stops)."""
+        return Server(self, ServeConfig(max_active, windows_per_step, hbm_budget_bytes,
+                                        large_capacity, mr_per_step))
+
     def verify_gaps(self, p_after, gaps):
         pa = np.ascontiguousarray(p_after, dtype=np.uint64)
         g = np.ascontiguousarray(gaps, dtype=np.int32)
@@ -354,3 +374,73 @@ class Engine:
         _check(lib().voss_verify_gaps(self._h, pa.ctypes.data, g.ctypes.data, pa.shape[0],
                                       ok.ctypes.data))
         return ok
+
+
+class Server:
+    """voss_serve_*: submit interval / MR requests from any thread; results
+    are identical to running each request alone."""
+
+    def __init__(self, engine, cfg):
+        self._h = ctypes.c_void_p()
+        self.large_capacity = cfg.large_capacity or 64
+        _check(lib().voss_serve_start(engine.handle, ctypes.byref(cfg), ctypes.byref(self._h)))
+        self._mr_sizes = {}
+
+    def submit_interval(self, lo, hi):
+        rid = ctypes.c_uint64()
+        _check(lib().voss_serve_submit_interval(self._h, lo, hi, ctypes.byref(rid)))
+        return rid.value
+
+    def submit_mr(self, ns):
+        ns = np.ascontiguousarray(ns, dtype=np.uint64)
+        rid = ctypes.c_uint64()
+        _check(lib().voss_serve_submit_mr(self._h, ns.ctypes.data, ns.shape[0], ctypes.byref(rid)))
+        self._mr_sizes[rid.value] = ns.shape[0]
+        return rid.value
+
+    def _out(self, rid):
+        if rid in self._mr_sizes:
+            v = np.zeros(max(self._mr_sizes[rid], 1), dtype=np.uint8)
+            return None, v, None
+        r = IntervalResult()
+        lbuf = (LargeGap * self.large_capacity)()
+        r.large = ctypes.cast(lbuf, ctypes.POINTER(LargeGap))
+        r.large_capacity = self.large_capacity
+        return r, None, lbuf
+
+    def _conv(self, rid, r, v):
+        if rid in self._mr_sizes:
+            return v[: self._mr_sizes.pop(rid)]
+        return interval_to_dict(r)
+
+    def wait(self, rid):
+        """(result, latency_s): dict for intervals, verdict array for MR."""
+        r, v, lbuf = self._out(rid)
+        lat = ctypes.c_double()
+        _check(lib().voss_serve_wait(self._h, rid, ctypes.byref(r) if r is not None else None,
+                                     v.ctypes.data if v is not None else None, ctypes.byref(lat)))
+        return self._conv(rid, r, v), lat.value
+
+    def poll(self, rid):
+        """None while running, else (result, latency_s)."""
+        r, v, lbuf = self._out(rid)
+        done = ctypes.c_int()
+        lat = ctypes.c_double()
+        _check(lib().voss_serve_poll(self._h, rid, ctypes.byref(done),
+                                     ctypes.byref(r) if r is not None else None,
+                                     v.ctypes.data if v is not None else None, ctypes.byref(lat)))
+        if not done.value:
+            return None
+        return self._conv(rid, r, v), lat.value
+
+    def stop(self):
+        if self._h:
+            rc = lib().voss_serve_stop(self._h)
+            self._h = ctypes.c_void_p()
+            _check(rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()
