@@ -38,6 +38,11 @@ CPU_SAMPLE_N = 10**10
 
 # MI355X LDS: 160 KB per CU, 128 bytes/clock per CU, 256 CUs at 2.4 GHz.
 LDS_PEAK_GBS = 128 * 256 * 2.4e9 / 1e9
+# HBM bytes per sieve_kernel launch at N=1e11 from the two rocprofv3 --pmc
+# runs (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, scaled by a 1 GiB copy kernel);
+# the commands and raw counters are in
+# profiles/sieve_kernel_plain_1e11_hbm_traffic.md.
+SIEVE_1E11_HBM_BYTES = (2 * 3897.375 + 17964.1875) * 1024
 
 
 def model_lds_bytes(N):
@@ -60,6 +65,28 @@ def model_lds_bytes(N):
     marks_big = float(np.sum(np.maximum(N / big - big, 0) * 8.0 / 30.0))
     m_eff = len(small) * words + marks_big
     return 8.0 * m_eff + 2.0 * (N / 30.0), m_eff
+
+
+def measured_lds_peaks(copy_bytes=0):
+    """LDS microbenchmarks from libvoss_probe.so (and the optional copy)."""
+    import ctypes
+    path = os.path.join(PKG, "lib", "libvoss_probe.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    L.voss_probe_lds.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.voss_probe_copy.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    if copy_bytes:
+        ms = ctypes.c_double()
+        L.voss_probe_copy(copy_bytes, ctypes.byref(ms))
+        out["copy_bytes"] = copy_bytes
+        out["copy_gbs"] = 2 * copy_bytes / (ms.value * 1e-3) / 1e9
+    for k, name in ((0, "ds_read_b128_gbs"), (1, "ds_and_random_gops"), (2, "ds_and_linear_gops")):
+        r = ctypes.c_double()
+        if L.voss_probe_lds(k, 256, ctypes.byref(r)) == 0:
+            out[name] = r.value / 1e9
+    return out
 
 
 def cpu_baseline(threads):
@@ -201,6 +228,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mr-count", type=int, default=10**7)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run one plain device copy of BYTES (the HBM-counter "
+                         "calibration kernel copy_b128 for rocprofv3 --pmc runs)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the 1e11 steps and MR (for profiling runs)")
     ap.add_argument("--no-plain", action="store_true",
@@ -215,6 +245,7 @@ def main():
     import vossgpu
 
     eng = vossgpu.Engine(0)
+    probe = measured_lds_peaks(args.copy_calibration)
     for _ in range(args.warmup):
         eng.run_plain(N_BENCH, SEG_NUM)
     step_s, sieve_ms, dev_ms = [], [], []
@@ -278,10 +309,11 @@ def main():
                    "windows": r["windows"], "device": eng.name},
         "roofline": {"bound": "lds", "kernel": "sieve_kernel",
                      "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / LDS_PEAK_GBS, "traffic": None,
+                     "frac": achieved / LDS_PEAK_GBS, "traffic": SIEVE_1E11_HBM_BYTES,
                      "algorithmic_bytes": lds_bytes, "m_eff": m_eff,
                      "avg_kernel_ms": statistics.mean(sieve_ms)},
         "device_ms_per_step": statistics.mean(dev_ms),
+        "lds_probe": probe,
         "mr": mr,
         "legs": legs,
     }

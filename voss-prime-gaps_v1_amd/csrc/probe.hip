@@ -55,7 +55,43 @@ __global__ __launch_bounds__(1024) void lds_and_linear(uint32_t iters, uint32_t*
     out[blockIdx.x * blockDim.x + threadIdx.x] = buf[threadIdx.x];
 }
 
+// Plain device copy: reads and writes `words` 16-byte words once each (the
+// HBM-traffic calibration for the FETCH_SIZE / WRITE_SIZE counters).
+__global__ __launch_bounds__(256) void copy_b128(const uint4* __restrict__ src,
+                                                 uint4* __restrict__ dst, size_t words) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+// One copy of `bytes` (multiple of 16) between two fresh device buffers.
+extern "C" int voss_probe_copy(size_t bytes, double* ms_out) {
+    uint4 *src = nullptr, *dst = nullptr;
+    if (hipMalloc((void**)&src, bytes) != hipSuccess) return -1;
+    if (hipMalloc((void**)&dst, bytes) != hipSuccess) {
+        (void)hipFree(src);
+        return -1;
+    }
+    (void)hipMemset(src, 1, bytes);
+    (void)hipDeviceSynchronize();
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(copy_b128, dim3(4096), dim3(256), 0, 0, src, dst, bytes / 16);
+    (void)hipEventRecord(e1, 0);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms_out) *ms_out = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 // kind 0: read b128 -> bytes/s; 1: ds_and random -> ops/s; 2: ds_and linear.
 extern "C" int voss_probe_lds(int kind, int num_cus, double* rate) {

@@ -100,9 +100,11 @@ constexpr int kLgtFirst = 51, kLgtCount = 200;
 
 struct PairCtx {
     uint32_t* hist;
-    uint32_t* mq;
     const unsigned long long* lgt;
-    const SieveArgs* a;
+    JobAcc* acc;                  // this window's job
+    LargeRaw* large;
+    unsigned long long* large_n;
+    unsigned long long large_cap;
     unsigned long long base_num;
     uint32_t job;
     uint32_t g_maybe;  // below it no gap of this window can be large
@@ -110,24 +112,25 @@ struct PairCtx {
 
 // A consecutive pair inside a window with gap g >= 100 (pb = offset of
 // p_before from base_num).  Gaps below 100 only touch the q=30 row
-// (mq[64*row + g/2]); the q=6 rows and the main histogram follow at flush.
-__device__ __noinline__ void pair_wide(const PairCtx& c, uint32_t pb, uint32_t g) {
+// (mq[8*(g/2) + row]); the q=6 rows and the main histogram follow at flush.
+// Inlined on purpose: a call would have to spill the context to scratch.
+__device__ __forceinline__ void pair_wide(const PairCtx& c, uint32_t pb, uint32_t g) {
     if (g >= 2u * kHistBins) {
-        atomicOr(&c.a->acc[c.job].error, 1ull);
+        atomicOr(&c.acc->error, 1ull);
         return;
     }
     atomicAdd(&c.hist[g >> 1], 1u);
     if (g >= c.g_maybe) {
-        unsigned long long p_before = c.base_num + pb;
+        const unsigned long long p_before = c.base_num + pb;
         if (g >= (uint32_t)kLargeAlways || p_before <= c.lgt[(g >> 1) - kLgtFirst]) {
-            unsigned long long slot = atomicAdd(c.a->large_n, 1ull);
-            atomicAdd(&c.a->acc[c.job].large_count, 1ull);
-            if (slot < c.a->large_cap) {
+            const unsigned long long slot = atomicAdd(c.large_n, 1ull);
+            atomicAdd(&c.acc->large_count, 1ull);
+            if (slot < c.large_cap) {
                 LargeRaw r;
                 r.p_before = p_before;
                 r.gap = g;
                 r.job = c.job;
-                c.a->large[slot] = r;
+                c.large[slot] = r;
             }
         }
     }
@@ -295,7 +298,8 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             uint32_t rel = (uint32_t)(D.n_hi - base_num);
             bithi = (int)((rel / 30u) * 8u) + c_lastbit[rel % 30u];
         }
-        PairCtx pc{hist, mq, s_lgt, &a, base_num, D.job, s_gmaybe};
+        const PairCtx pc{hist, s_lgt, &a.acc[D.job], a.large, a.large_n, a.large_cap,
+                         base_num, D.job, s_gmaybe};
         const uint32_t per = L4 / BLOCK;
         const uint32_t i0 = (uint32_t)tid * per;
         const uint32_t iend = i0 + per;
