@@ -101,33 +101,67 @@ def cpu_baseline(threads):
                       f"{threads} threads"}
 
 
-def batch_leg(eng, W, with_plain=True):
+def batch_leg(eng, W, with_plain=True, autotune=True):
     """Config 4: 4096 seeded intervals (starts in [1e12, 1e16), lengths
-    log-uniform in [1e4, 1e9]) in one persistent launch sequence, against
-    one launch sequence per interval; outputs compared bit for bit."""
+    log-uniform in [1e4, 1e9]).  The autotuner picks window, workgroup and
+    bucket thresholds on a 256-interval sample; then the batch runs as one
+    captured hipGraph (one call, planning included), as replays of a
+    prepared batch, eagerly (for the per-kernel split), and as one launch
+    sequence per interval; all outputs are compared bit for bit."""
+    import vossgpu
     iv = W.batch_intervals(4096)
     lo = [a for a, _ in iv]
     hi = [b for _, b in iv]
     numbers = sum(b - a for a, b in iv)
     eng.run_interval(10**16, 10**16 + 10**6)  # base primes to 1e8: outside the timing
-    t = time.perf_counter()
-    rb, lbuf, st = eng.run_batch(lo, hi, large_cap=64, raw=True)
-    t_batch = time.perf_counter() - t
-    out = {"intervals": len(iv), "numbers": numbers, "batched_s": t_batch,
-           "batched_numbers_per_s": numbers / t_batch, "sieve_ms": st["sieve_ms"],
-           "bucket_ms": st["bucket_ms"], "windows": st["windows"]}
-    if with_plain:
+    default = eng.tuning()
+    out = {"intervals": len(iv), "numbers": numbers}
+    if autotune:
         t = time.perf_counter()
-        rp, lbuf2, sp = eng.run_batch_plain(lo, hi, large_cap=64, raw=True)
-        t_plain = time.perf_counter() - t
-        import vossgpu
-        off = vossgpu.IntervalResult.large.offset  # compare all fields but the pointer
+        best, sec = eng.autotune(lo, hi, max_intervals=256, repeats=1)
+        out["autotune"] = {"tuning": best, "sample_intervals": 256, "sample_s": sec,
+                           "tuner_s": time.perf_counter() - t}
+    off = vossgpu.IntervalResult.large.offset  # compare all fields but the pointer
 
-        def lg(r):
-            return vossgpu.large_array(r.large, min(r.large_count, r.large_capacity)).tobytes()
-        same = all(bytes(a)[:off] == bytes(b)[:off] and lg(a) == lg(b) for a, b in zip(rb, rp))
-        out.update({"plain_s": t_plain, "batched_over_plain": t_plain / t_batch,
-                    "bit_exact": bool(same)})
+    def lg(r):
+        return vossgpu.large_array(r.large, min(r.large_count, r.large_capacity)).tobytes()
+
+    def same(ra, rb):
+        return all(bytes(a)[:off] == bytes(b)[:off] and lg(a) == lg(b) for a, b in zip(ra, rb))
+
+    try:
+        eng.run_batch(lo, hi, large_cap=64, raw=True)  # warm-up (buffers sized)
+        t = time.perf_counter()
+        rb, lbuf, st = eng.run_batch(lo, hi, large_cap=64, raw=True)
+        t_batch = time.perf_counter() - t
+        out.update({"batched_s": t_batch, "batched_numbers_per_s": numbers / t_batch,
+                    "device_ms": st["device_ms"], "windows": st["windows"],
+                    "bucket_room_entries": st["bucket_entries"]})
+        with eng.prepare_batch(lo, hi) as pb:
+            pb.run(large_cap=64, raw=True)  # capture
+            reps = []
+            for _ in range(3):
+                t = time.perf_counter()
+                rr, lbuf3, _ = pb.run(large_cap=64, raw=True)
+                reps.append(time.perf_counter() - t)
+            out.update({"replay_s": min(reps), "replay_numbers_per_s": numbers / min(reps),
+                        "replay_bit_exact": bool(same(rb, rr))})
+        eng.set_graph_mode(False)
+        try:
+            t = time.perf_counter()
+            re_, lbuf4, se = eng.run_batch(lo, hi, large_cap=64, raw=True)
+            out.update({"eager_s": time.perf_counter() - t, "sieve_ms": se["sieve_ms"],
+                        "bucket_ms": se["bucket_ms"], "eager_bit_exact": bool(same(rb, re_))})
+        finally:
+            eng.set_graph_mode(True)
+        if with_plain:
+            t = time.perf_counter()
+            rp, lbuf2, sp = eng.run_batch_plain(lo, hi, large_cap=64, raw=True)
+            t_plain = time.perf_counter() - t
+            out.update({"plain_s": t_plain, "batched_over_plain": t_plain / t_batch,
+                        "bit_exact": bool(same(rb, rp))})
+    finally:
+        eng.set_tuning(**default)
     # gap endpoints for the Miller-Rabin leg
     ends = []
     for k in range(len(iv)):

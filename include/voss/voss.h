@@ -113,7 +113,11 @@ typedef struct {
                                (jobs of >= bucket_min_windows windows);
                                UINT32_MAX disables buckets */
     uint32_t bucket_min_windows;
-    uint32_t reserved[3];
+    uint32_t bucket_room_pct; /* room of a bucket as a percentage of its
+                                 estimate (default 100: mean + 8 sigma);
+                                 a bucket that outgrows it makes the run
+                                 repeat with exact counts */
+    uint32_t reserved[2];
 } voss_tuning;
 
 /* ---------------------------------------------------------------- context */
@@ -127,6 +131,16 @@ int voss_device_name(voss_ctx* ctx, char* buf, size_t cap, int* num_cus,
                      double* total_gb);
 int voss_set_tuning(voss_ctx* ctx, const voss_tuning* t);
 int voss_get_tuning(voss_ctx* ctx, voss_tuning* t);
+
+/* Autotuner: coordinate descent over window_bytes, tier_a_limit,
+ * bucket_limit and bucket_min_windows, timing voss_run_batch on the given
+ * intervals (at most max_intervals of them, evenly spread) with `repeats`
+ * timed runs per candidate after one warm-up.  Installs the fastest tuning
+ * on the context and returns it in best (and its time in best_seconds).
+ * Results never depend on the tuning, only the time does. */
+int voss_autotune(voss_ctx* ctx, const uint64_t* lo, const uint64_t* hi, size_t n,
+                  size_t max_intervals, int repeats, voss_tuning* best,
+                  double* best_seconds);
 
 /* ------------------------------------------------------------------ sieve */
 
@@ -174,6 +188,27 @@ int voss_run_batch(voss_ctx* ctx, const uint64_t* lo, const uint64_t* hi,
 int voss_run_batch_plain(voss_ctx* ctx, const uint64_t* lo,
                          const uint64_t* hi, size_t n,
                          voss_interval_result* out, voss_result* stats);
+
+/* enable = 1 (the default): voss_run_batch captures its whole device
+ * sequence (bucket fills, sieves, stitches of every round, read-back) in
+ * one hipGraph and launches that.  enable = 0: ordinary stream launches,
+ * which also time the bucket and sieve kernels separately (stats->sieve_ms
+ * and stats->bucket_ms; under a graph they read -1). */
+int voss_set_graph_mode(voss_ctx* ctx, int enable);
+
+/* Prepared batch (no reference counterpart; for callers that sieve the
+ * same interval set repeatedly).  voss_batch_prepare plans the windows,
+ * rounds and bucket tasks once and gives the batch its own device buffers;
+ * the first voss_batch_run captures the device sequence in a hipGraph and
+ * every later run replays it.  The context's tuning must not change in
+ * between (voss_batch_run then fails with VOSS_EINVAL).  Results are those
+ * of voss_run_batch on the same intervals. */
+typedef struct voss_batch voss_batch;
+int voss_batch_prepare(voss_ctx* ctx, const uint64_t* lo, const uint64_t* hi,
+                       size_t n, voss_batch** out);
+int voss_batch_run(voss_batch* batch, voss_interval_result* out,
+                   voss_result* stats);
+void voss_batch_free(voss_batch* batch);
 
 /* ----------------------------------------------------------- Miller-Rabin */
 

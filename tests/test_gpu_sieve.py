@@ -265,3 +265,110 @@ def test_large_gap_truncation_is_canonical(engine, oracle):
     (p0, p1), _ = engine.run_batch_plain(lo2, hi2, large_cap=8)
     assert b0["large"].tolist() == top == p0["large"].tolist()
     assert b1["large"].tolist() == p1["large"].tolist()
+
+
+def test_autotune_installs_a_valid_tuning_and_keeps_results(engine, oracle):
+    """The autotuner only changes timing: the batch after tuning is the
+    oracle's, and the installed tuning is one of the searched shapes."""
+    iv = seeded_intervals(32, 0x564F5353 + 11, lo_min=10**12, lo_max=10**15, lmin=4, lmax=8)
+    lo = [a for a, _ in iv]
+    hi = [b for _, b in iv]
+    old = engine.tuning()
+    try:
+        print("[gpu] autotune on 32 intervals (sample 16)", flush=True)
+        t, sec = engine.autotune(lo, hi, max_intervals=16, repeats=1)
+        assert sec > 0
+        assert (t["window_bytes"], t["wg_per_cu"]) in {(131072, 1), (65536, 2), (65536, 1),
+                                                       (32768, 4)}
+        assert t["bucket_limit"] >= t["window_bytes"]
+        assert engine.tuning() == t
+        batch, _ = engine.run_batch(lo, hi, large_cap=4096)
+        for (a, b), rb in zip(iv, batch):
+            assert_same(rb, oracle.run_interval(a, b))
+    finally:
+        engine.set_tuning(**old)
+
+
+def _mixed_batch(seed):
+    """Short and long intervals, the long ones through bucket lists."""
+    iv = seeded_intervals(20, seed, lo_min=10**12, lo_max=10**15, lmin=4, lmax=7)
+    iv += seeded_intervals(6, seed + 1, lo_min=10**14, lo_max=10**16, lmin=7.2, lmax=8.3)
+    iv += [(0, 1000), (10**6, 10**6), (7, 8)]
+    return [a for a, _ in iv], [b for _, b in iv]
+
+
+def test_graph_mode_matches_eager_and_oracle(engine, oracle):
+    lo, hi = _mixed_batch(0x564F5353 + 21)
+    print(f"[gpu] graph vs eager batch of {len(lo)}", flush=True)
+    try:
+        engine.set_graph_mode(True)
+        g, gst = engine.run_batch(lo, hi, large_cap=4096)
+        engine.set_graph_mode(False)
+        e, est = engine.run_batch(lo, hi, large_cap=4096)
+    finally:
+        engine.set_graph_mode(True)
+    assert gst["sieve_ms"] == -1 and est["sieve_ms"] > 0
+    assert gst["bucket_entries"] > 0
+    for a, b, rg, re_ in zip(lo, hi, g, e):
+        assert_same(rg, re_)
+    for k in range(0, len(lo), 4):
+        assert_same(g[k], oracle.run_interval(lo[k], hi[k]))
+
+
+def test_prepared_batch_replays(engine):
+    """Three replays of one captured graph, with other eager work and
+    synchronisations between them; each equals the eager batch."""
+    lo, hi = _mixed_batch(0x564F5353 + 23)
+    engine.set_graph_mode(False)
+    try:
+        ref, _ = engine.run_batch(lo, hi, large_cap=512)
+    finally:
+        engine.set_graph_mode(True)
+    with engine.prepare_batch(lo, hi) as pb:
+        for rep in range(3):
+            print(f"[gpu] prepared batch replay {rep}", flush=True)
+            got, st = pb.run(large_cap=512)
+            assert st["device_ms"] > 0
+            for rg, rr in zip(got, ref):
+                assert_same(rg, rr)
+            # eager work on the same context between replays
+            engine.run_interval(10**13 + rep * 10**9, 10**13 + rep * 10**9 + 10**8)
+            engine.run_batch(lo[:3], hi[:3])
+
+
+def test_prepared_batch_rejects_changed_tuning(engine):
+    old = engine.tuning()
+    with engine.prepare_batch([10**12], [10**12 + 10**6]) as pb:
+        try:
+            engine.set_tuning(tier_a_limit=old["tier_a_limit"] * 2)
+            with pytest.raises(Exception, match="tuning changed"):
+                pb.run()
+        finally:
+            engine.set_tuning(**old)
+        got, _ = pb.run()
+        assert got[0]["total_primes"] > 0
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_bucket_overflow_reruns_exactly(engine, oracle, graph):
+    """Rooms at 50% of the estimate: every bucket overflows, the fill kernel
+    flags it, and the run repeats with exact counts.  Compared with the
+    default run (pinned to the oracle by the graph-mode test) and, for two
+    long intervals, with the oracle itself."""
+    lo, hi = _mixed_batch(0x564F5353 + 25)
+    ref, _ = engine.run_batch(lo, hi, large_cap=4096)
+    old = engine.tuning()
+    try:
+        engine.set_graph_mode(graph)
+        engine.set_tuning(bucket_room_pct=50)
+        print(f"[gpu] bucket overflow rerun graph={graph}", flush=True)
+        got, st = engine.run_batch(lo, hi, large_cap=4096)
+    finally:
+        engine.set_tuning(**old)
+        engine.set_graph_mode(True)
+    assert st["sieve_ms"] > 0  # the exact re-run is timed per kernel
+    for rg, rr in zip(got, ref):
+        assert_same(rg, rr)
+    longest = sorted(range(len(lo)), key=lambda k: hi[k] - lo[k])[-2:]
+    for k in longest:
+        assert_same(got[k], oracle.run_interval(lo[k], hi[k]))

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -81,6 +82,37 @@ struct HostBuf {
     }
 };
 
+// Buffers of one execution of a plan, device and pinned host.  The context
+// owns one set for its calls; a prepared batch owns its own, so its
+// captured graph keeps valid pointers between replays.
+struct ExecBufs {
+    DevBuf<SegDesc> d_desc;
+    DevBuf<SegRec> d_rec;
+    DevBuf<JobAcc> d_acc;
+    DevBuf<LargeRaw> d_large;
+    DevBuf<unsigned long long> d_flags;  // [0] large-gap records, [1] bucket overflow
+    DevBuf<BucketTask> d_btask;
+    DevBuf<uint32_t> d_bcnt;
+    DevBuf<uint32_t> d_bcur;
+    DevBuf<unsigned long long> d_boff;
+    DevBuf<uint32_t> d_bent;
+    HostBuf<SegDesc> h_desc;
+    HostBuf<BucketTask> h_btask;
+    HostBuf<unsigned long long> h_boff;
+    HostBuf<uint32_t> h_bcnt;
+    HostBuf<SegRec> h_rec;
+    HostBuf<JobAcc> h_acc;
+    HostBuf<LargeRaw> h_large;
+    HostBuf<unsigned long long> h_flags;
+    void release() {
+        d_desc.release(); d_rec.release(); d_acc.release(); d_large.release();
+        d_flags.release(); d_btask.release(); d_bcnt.release(); d_bcur.release();
+        d_boff.release(); d_bent.release(); h_desc.release(); h_btask.release();
+        h_boff.release(); h_bcnt.release(); h_rec.release(); h_acc.release();
+        h_large.release(); h_flags.release();
+    }
+};
+
 constexpr uint32_t kW30Mask[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 
 uint64_t isqrt_u64(uint64_t n) {
@@ -141,7 +173,7 @@ struct voss_ctx {
     std::string name;
     int num_cus = 0;
     double total_gb = 0;
-    voss_tuning tune{131072, 1, 1024, 1u << 21, 4, {0, 0, 0}};
+    voss_tuning tune{131072, 1, 1024, 1u << 21, 4, 100, {0, 0}};
 
     std::vector<uint32_t> primes;
     std::vector<double> inv_prefix;  // inv_prefix[i] = sum of 1/p over primes[0..i)
@@ -157,19 +189,9 @@ struct voss_ctx {
     std::vector<unsigned long long> lg_thresh;
     DevBuf<unsigned long long> d_lg_thresh;
 
-    DevBuf<SegDesc> d_desc;
-    DevBuf<SegRec> d_rec;
-    DevBuf<JobAcc> d_acc;
-    DevBuf<LargeRaw> d_large;
-    DevBuf<unsigned long long> d_large_n;
-
-    DevBuf<BucketTask> d_btask;
-    DevBuf<uint32_t> d_bcnt;
-    DevBuf<uint32_t> d_bcur;
-    DevBuf<unsigned long long> d_boff;
-    DevBuf<uint32_t> d_bent;
-    HostBuf<uint32_t> h_bcnt;
-    HostBuf<unsigned long long> h_boff;
+    ExecBufs eb;                  // buffers of the context's own runs
+    std::vector<hipEvent_t> evs;  // per-round timing events
+    int graph_mode = 1;           // batched runs as one captured hipGraph
 
     // per-window prime counts of the last plain run (gap-sequence sampling)
     uint64_t last_N = 0, last_seg = 0;
@@ -177,12 +199,6 @@ struct voss_ctx {
     std::vector<uint64_t> last_counts;
     DevBuf<unsigned long long> d_emit;
     DevBuf<unsigned long long> d_emit_off;
-
-    HostBuf<SegDesc> h_desc;
-    HostBuf<SegRec> h_rec;
-    HostBuf<JobAcc> h_acc;
-    HostBuf<LargeRaw> h_large;
-    HostBuf<unsigned long long> h_large_n;
 
     DevBuf<uint64_t> d_mr_n;
     DevBuf<uint8_t> d_mr_v;
@@ -448,9 +464,9 @@ void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job
     }
 }
 
-SieveArgs make_args(voss_ctx* c, uint32_t ndesc, unsigned long long large_cap) {
+SieveArgs make_args(voss_ctx* c, ExecBufs& B, uint32_t ndesc, unsigned long long large_cap) {
     SieveArgs a{};
-    a.desc = c->d_desc.p;
+    a.desc = B.d_desc.p;
     a.ndesc = ndesc;
     a.tier_a_begin = kPresievePrimes;
     a.tier_a_end = (uint32_t)(std::lower_bound(c->primes.begin(), c->primes.end(),
@@ -466,44 +482,59 @@ SieveArgs make_args(voss_ctx* c, uint32_t ndesc, unsigned long long large_cap) {
         a.pat_step[g] = (uint32_t)block % c->pat_len[g];
     }
     a.lg_thresh = c->d_lg_thresh.p;
-    a.bucket_off = c->d_boff.p;
-    a.bucket_cnt = c->d_bcnt.p;
-    a.bucket_ent = c->d_bent.p;
+    a.bucket_off = B.d_boff.p;
+    a.bucket_cnt = B.d_bcur.p;
+    a.bucket_ent = B.d_bent.p;
     a.emit_out = c->d_emit.p;
     a.emit_off = c->d_emit_off.p;
-    a.rec = c->d_rec.p;
-    a.acc = c->d_acc.p;
-    a.large = c->d_large.p;
+    a.rec = B.d_rec.p;
+    a.acc = B.d_acc.p;
+    a.large = B.d_large.p;
     a.large_cap = large_cap;
-    a.large_n = c->d_large_n.p;
+    a.large_n = B.d_flags.p;
     return a;
 }
 
 constexpr size_t kMaxDescPerLaunch = 1u << 20;
 
-// Runs the plan: sieve + stitch per launch of at most kMaxDescPerLaunch
-// windows, joining launch boundaries on the host.  jobs[j] receives the
-// accumulated result of job j.  large_cap_total bounds the device list.
 struct ExecTimes {
-    double device_ms = 0;  // buckets + sieve + stitch
-    double sieve_ms = 0;   // sieve kernel alone
-    double bucket_ms = 0;  // bucket count + fill kernels
+    double device_ms = 0;  // the whole device sequence
+    double sieve_ms = 0;   // sieve kernels (-1: not timed, under a graph)
+    double bucket_ms = 0;  // bucket kernels (-1: not timed, under a graph)
     uint64_t bucket_entries = 0;
     uint32_t rounds = 0;
+    uint32_t reruns = 0;   // re-executions after a bucket or large-list overflow
 };
 
-constexpr uint64_t kBucketEntryBudget = 1ull << 30;  // entries per round (4 GB)
+// Bucket entries per round (8 GiB of HBM); offsets stay 32-bit.
+constexpr uint64_t kBucketEntryBudget = 1ull << 31;
 // Primes per bucket task: long tiles give every thread many hits already;
 // short tiles need more primes per task to cover the task's fixed costs.
 uint32_t bucket_chunk(uint32_t nwin) { return nwin >= 256 ? 512u : (nwin >= 32 ? 2048u : 8192u); }
 
-// Splits the plan into rounds (consecutive windows) and, per round, gives
-// the windows of long jobs a bucket for the primes >= bucket_limit.
+// Room for the entries of one bucket (a window of W bytes) fed by primes
+// with sum(1/p) = s.  A prime p has on average 8W/p multiples coprime to 30
+// among the window's 30W numbers; the window's count is a sum of nearly
+// independent per-prime counts whose variance is at most their mean, so
+// mean + 8 sigma + 64 is essentially never exceeded.  If it is, the fill
+// kernel reports it and the plan is re-run with exact counts.
+uint32_t bucket_room(double s, uint32_t W, uint32_t pct) {
+    const double e = 8.0 * W * s;
+    const double r = (e + 8.0 * std::sqrt(e) + 64.0) * pct / 100.0;
+    return (uint32_t)(((uint64_t)std::ceil(r) + 15) / 16 * 16);
+}
+
+// A round is a run of consecutive windows of the plan launched together:
+// bucket fill, sieve, stitch.  Windows of jobs with at least
+// bucket_min_windows windows get a bucket for the primes >= bucket_limit.
 struct Round {
-    size_t off, nd;
+    size_t off = 0, nd = 0;        // plan windows [off, off + nd)
     std::vector<SegDesc> desc;     // with bucket / nprimes set for this round
     std::vector<BucketTask> tasks;
     uint32_t nbuckets = 0;
+    std::vector<uint32_t> room;    // entries per bucket
+    size_t task_off = 0, bkt_off = 0, boff_off = 0;  // places in the staged arrays
+    uint64_t room_total = 0;
 };
 
 void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
@@ -521,13 +552,14 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
     while (off < P.desc.size()) {
         Round R;
         R.off = off;
-        double est = 0;
+        uint64_t est = 0;
         size_t end = off;
         while (end < P.desc.size() && end - off < kMaxDescPerLaunch) {
             const SegDesc& d = P.desc[end];
             if (bucketed(d)) {
-                const double e = 8.0 * d.len * (c->inv_prefix[d.nprimes] - c->inv_prefix[ibl]);
-                if (end > off && est + e > 0.7 * (double)kBucketEntryBudget) break;
+                const uint64_t e = bucket_room(c->inv_prefix[d.nprimes] - c->inv_prefix[ibl], W,
+                                               c->tune.bucket_room_pct);
+                if (end > off && est + e > kBucketEntryBudget * 15 / 16) break;
                 est += e;
             }
             end++;
@@ -557,17 +589,33 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
                 j++;
             }
             const uint32_t nwin = R.nbuckets - nb0;
+            const uint32_t room = bucket_room(c->inv_prefix[np_max] - c->inv_prefix[ibl], W,
+                                             c->tune.bucket_room_pct);
+            R.room.insert(R.room.end(), nwin, room);
+            R.room_total += (uint64_t)room * nwin;
             for (uint32_t t0 = 0; t0 < nwin; t0 += kBucketTileMax) {
                 const uint32_t tn = std::min<uint32_t>(kBucketTileMax, nwin - t0);
-                const uint32_t chunk = bucket_chunk(tn);
-                for (size_t p0 = ibl; p0 < np_max; p0 += chunk) {
-                    BucketTask T;
-                    T.w0 = d.w0 + (uint64_t)t0 * W;
-                    T.b0 = nb0 + t0;
-                    T.nwin = tn;
-                    T.i0 = (uint32_t)p0;
-                    T.i1 = (uint32_t)std::min<size_t>(np_max, p0 + chunk);
-                    R.tasks.push_back(T);
+                // primes beyond 1.5x the tile length: walk mode
+                const uint64_t walk_from = std::min<uint64_t>(3ull * tn * W / 2, 0xFFFFFFFFu);
+                const size_t iw = std::min<size_t>(
+                    np_max, std::max<size_t>(ibl, (size_t)(std::lower_bound(
+                                                               c->primes.begin(), c->primes.end(),
+                                                               (uint32_t)walk_from) -
+                                                           c->primes.begin())));
+                for (int walk = 0; walk < 2; walk++) {
+                    const size_t a0 = walk ? iw : ibl, a1 = walk ? np_max : iw;
+                    const uint32_t chunk = bucket_chunk(tn) * (walk ? 8u : 1u);
+                    for (size_t p0 = a0; p0 < a1; p0 += chunk) {
+                        BucketTask T;
+                        T.w0 = d.w0 + (uint64_t)t0 * W;
+                        T.b0 = nb0 + t0;
+                        T.nwin = tn;
+                        T.i0 = (uint32_t)p0;
+                        T.i1 = (uint32_t)std::min<size_t>(a1, p0 + chunk);
+                        T.walk = (uint32_t)walk;
+                        T.pad = 0;
+                        R.tasks.push_back(T);
+                    }
                 }
             }
             i = j;
@@ -577,162 +625,376 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
     }
 }
 
-// Bucket count -> host prefix sum -> bucket fill for one round.
-int run_buckets(voss_ctx* c, const Round& R, ExecTimes* tm) {
-    if (R.tasks.empty()) return VOSS_OK;
-    int log2w = 0;
-    while ((1u << log2w) < c->tune.window_bytes) log2w++;
-    const uint32_t grid = (uint32_t)c->num_cus * 2;
-    HIPCHK(c->d_btask.reserve(R.tasks.size()));
-    HIPCHK(c->d_bcnt.reserve(R.nbuckets));
-    HIPCHK(c->d_bcur.reserve(R.nbuckets));
-    HIPCHK(c->d_boff.reserve(R.nbuckets));
-    HIPCHK(c->h_bcnt.reserve(R.nbuckets));
-    HIPCHK(c->h_boff.reserve(R.nbuckets));
-    HIPCHK(hipMemcpyAsync(c->d_btask.p, R.tasks.data(), R.tasks.size() * sizeof(BucketTask),
-                          hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_fill_u32(c->d_bcnt.p, R.nbuckets, 0, c->stream));
-    HIPCHK(launch_fill_u32(c->d_bcur.p, R.nbuckets, 0, c->stream));
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(launch_bucket_count(c->d_btask.p, (uint32_t)R.tasks.size(), c->d_primes.p,
-                               c->d_bcnt.p, log2w, grid, c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_bcnt.p, c->d_bcnt.p, R.nbuckets * 4, hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    uint64_t total = 0;
-    for (uint32_t b = 0; b < R.nbuckets; b++) {
-        c->h_boff.p[b] = total;
-        total += c->h_bcnt.p[b];
-    }
-    if (total >= 0xFFFFFFFFull) return fail(VOSS_ENOMEM, "bucket round beyond 2^32 entries");
-    HIPCHK(c->d_bent.reserve(std::max<uint64_t>(total, 1)));
-    HIPCHK(hipMemcpyAsync(c->d_boff.p, c->h_boff.p, R.nbuckets * 8, hipMemcpyHostToDevice,
-                          c->stream));
-    HIPCHK(launch_bucket_fill(c->d_btask.p, (uint32_t)R.tasks.size(), c->d_primes.p,
-                              c->d_boff.p, c->d_bcur.p, c->d_bent.p, log2w, grid, c->stream));
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipEventSynchronize(c->ev1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    tm->bucket_ms += ms;
-    tm->bucket_entries += total;
-    return VOSS_OK;
+int log2_window(const voss_ctx* c) {
+    int l = 0;
+    while ((1u << l) < c->tune.window_bytes) l++;
+    return l;
 }
 
-int execute(voss_ctx* c, const Plan& P, std::vector<JobHost>& jobs,
-            uint64_t large_cap_total, ExecTimes* times,
-            std::vector<uint64_t>* counts = nullptr) {
-    jobs.assign(P.njobs, JobHost());
-    if (P.njobs == 0) return VOSS_OK;
-    HIPCHK(c->d_acc.reserve(P.njobs));
-    HIPCHK(c->d_large_n.reserve(1));
-    // The device keeps every large-gap candidate of a round, so that the
-    // caller's top-K (canonical order) does not depend on atomic order; a
-    // round that overflows the list is re-run with a larger one.
-    (void)large_cap_total;
-    uint64_t lcap = std::max<uint64_t>(c->d_large.cap, 1u << 20);
-    HIPCHK(c->d_large.reserve(lcap));
-    const uint32_t block_per_cu = std::max<uint32_t>(c->tune.wg_per_cu, 1);
-    const uint32_t grid = (uint32_t)c->num_cus * block_per_cu;
-
-    // carry across launch boundaries: last non-empty window per job
-    std::vector<uint64_t> carry_last(P.njobs, 0);
-    std::vector<uint32_t> carry_lseg(P.njobs, 0);
-    std::vector<uint8_t> carry_has(P.njobs, 0);
-    HIPCHK(c->h_acc.reserve(P.njobs));
-    HIPCHK(c->h_large_n.reserve(1));
-    float total_ms = 0, sieve_ms = 0;
-
+// Runs a plan on the device: every round's bucket fill, sieve and stitch
+// in stream order, then the accumulators, window records and flags back
+// to pinned memory, with a single synchronisation at the end.  The direct
+// sequence has no host step inside, so it can be captured in a hipGraph
+// and replayed.  Each job's accumulators sum over all its rounds on the
+// device; the pairs across round edges are joined on the host from the
+// window records.
+struct Executor {
+    voss_ctx* c = nullptr;
+    ExecBufs* B = nullptr;
+    Plan P;
     std::vector<Round> rounds;
-    make_rounds(c, P, rounds);
-    ExecTimes tmb;
-    for (size_t ri = 0; ri < rounds.size(); ri++) {
-        const Round& R = rounds[ri];
-        const size_t off = R.off;
-        const uint32_t nd = (uint32_t)R.nd;
-        int brc = run_buckets(c, R, &tmb);
-        if (brc) return brc;
-        HIPCHK(c->d_desc.reserve(nd));
-        HIPCHK(c->d_rec.reserve(nd));
-        HIPCHK(c->h_desc.reserve(nd));
-        HIPCHK(c->h_rec.reserve(nd));
-        std::memcpy(c->h_desc.p, R.desc.data(), nd * sizeof(SegDesc));
-        HIPCHK(hipMemcpyAsync(c->d_desc.p, c->h_desc.p, nd * sizeof(SegDesc),
-                              hipMemcpyHostToDevice, c->stream));
-        HIPCHK(launch_init_acc(c->d_acc.p, P.njobs, c->stream));
-        HIPCHK(hipMemsetAsync(c->d_large_n.p, 0, sizeof(unsigned long long), c->stream));
-        SieveArgs a = make_args(c, nd, lcap);
-        HIPCHK(hipEventRecord(c->ev0, c->stream));
-        HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, c->stream));
-        HIPCHK(hipEventRecord(c->ev2, c->stream));
-        HIPCHK(launch_stitch(a, c->stream));
-        HIPCHK(hipEventRecord(c->ev1, c->stream));
-        HIPCHK(hipMemcpyAsync(c->h_acc.p, c->d_acc.p, P.njobs * sizeof(JobAcc),
-                              hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(c->h_large_n.p, c->d_large_n.p, 8, hipMemcpyDeviceToHost,
-                              c->stream));
-        HIPCHK(hipMemcpyAsync(c->h_rec.p, c->d_rec.p, nd * sizeof(SegRec),
-                              hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        float ms = 0, sms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        HIPCHK(hipEventElapsedTime(&sms, c->ev0, c->ev2));
-        total_ms += ms;
-        sieve_ms += sms;
-        const JobAcc* acc = c->h_acc.p;
-        const SegRec* rec = c->h_rec.p;
-        if (c->h_large_n.p[0] > lcap) {
-            lcap = c->h_large_n.p[0] + (c->h_large_n.p[0] >> 2);
-            HIPCHK(c->d_large.reserve(lcap));
-            ri--;  // same round again with room for every candidate
-            continue;
+    size_t ntask = 0, nbkt = 0, nboff = 0;
+    uint64_t lcap = 0;         // device large-gap list capacity
+    bool staged = false;       // buffers sized and staging arrays filled
+    bool exact = false;        // bucket rooms are the exact counts
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const PrimeRec* graph_primes = nullptr;  // base-prime table the graph reads
+    bool copy_acc = true;      // read the accumulators back after a run
+    uint64_t nlarge = 0;       // large-gap records of the last run (in B->h_large)
+
+    Executor(voss_ctx* ctx, ExecBufs* bufs) : c(ctx), B(bufs) {}
+    ~Executor() { drop_graph(); }
+    Executor(const Executor&) = delete;
+    Executor& operator=(const Executor&) = delete;
+
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+
+    void plan_rounds() {
+        rounds.clear();
+        make_rounds(c, P, rounds);
+        ntask = nbkt = nboff = 0;
+        for (Round& R : rounds) {
+            R.task_off = ntask;
+            R.bkt_off = nbkt;
+            R.boff_off = nboff;
+            ntask += R.tasks.size();
+            nbkt += R.nbuckets;
+            nboff += R.nbuckets + 1;
         }
-        const uint64_t nkeep = c->h_large_n.p[0];
-        HIPCHK(c->h_large.reserve(nkeep));
-        if (nkeep) {
-            HIPCHK(hipMemcpyAsync(c->h_large.p, c->d_large.p, nkeep * sizeof(LargeRaw),
+        staged = false;
+        exact = false;
+        drop_graph();
+    }
+
+    // Sizes every buffer and fills the pinned staging arrays (descriptors,
+    // tasks, bucket offsets from the rooms).  Nothing is allocated after
+    // this until the next staging, so captured pointers stay valid.
+    int stage() {
+        const size_t nd = P.desc.size();
+        const uint32_t nj = std::max<uint32_t>(P.njobs, 1);
+        if (lcap == 0) lcap = std::max<uint64_t>(B->d_large.cap, 1u << 20);
+        uint64_t ent = 1;
+        uint32_t nb_max = 1;
+        for (const Round& R : rounds) {
+            ent = std::max<uint64_t>(ent, R.room_total);
+            nb_max = std::max<uint32_t>(nb_max, R.nbuckets);
+        }
+        if (ent >= 0xFFFFFFFFull) return fail(VOSS_ENOMEM, "bucket round beyond 2^32 entries");
+        drop_graph();
+        HIPCHK(B->d_desc.reserve(nd));
+        HIPCHK(B->d_rec.reserve(nd));
+        HIPCHK(B->d_acc.reserve(nj));
+        HIPCHK(B->d_large.reserve(lcap));
+        HIPCHK(B->d_flags.reserve(2));
+        HIPCHK(B->d_btask.reserve(ntask));
+        HIPCHK(B->d_bcnt.reserve(nb_max));
+        HIPCHK(B->d_bcur.reserve(nbkt));
+        HIPCHK(B->d_boff.reserve(nboff));
+        HIPCHK(B->d_bent.reserve(ent));
+        HIPCHK(B->h_desc.reserve(nd));
+        HIPCHK(B->h_btask.reserve(ntask));
+        HIPCHK(B->h_boff.reserve(nboff));
+        HIPCHK(B->h_bcnt.reserve(nb_max));
+        HIPCHK(B->h_rec.reserve(nd));
+        HIPCHK(B->h_acc.reserve(nj));
+        HIPCHK(B->h_flags.reserve(2));
+        for (const Round& R : rounds) {
+            std::memcpy(B->h_desc.p + R.off, R.desc.data(), R.nd * sizeof(SegDesc));
+            if (!R.tasks.empty())
+                std::memcpy(B->h_btask.p + R.task_off, R.tasks.data(),
+                            R.tasks.size() * sizeof(BucketTask));
+            unsigned long long o = 0;
+            for (uint32_t b = 0; b < R.nbuckets; b++) {
+                B->h_boff.p[R.boff_off + b] = o;
+                o += R.room[b];
+            }
+            B->h_boff.p[R.boff_off + R.nbuckets] = o;
+        }
+        staged = true;
+        return VOSS_OK;
+    }
+
+    hipEvent_t ev(size_t i) { return c->evs[i]; }
+
+    int ensure_events(size_t n) {
+        while (c->evs.size() < n) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c->evs.push_back(e);
+        }
+        return VOSS_OK;
+    }
+
+    SieveArgs round_args(const Round& R) {
+        SieveArgs a = make_args(c, *B, (uint32_t)R.nd, lcap);
+        a.desc = B->d_desc.p + R.off;
+        a.rec = B->d_rec.p + R.off;
+        a.bucket_off = B->d_boff.p + R.boff_off;
+        a.bucket_cnt = B->d_bcur.p + R.bkt_off;
+        return a;
+    }
+
+    // The direct sequence (no host step): capturable.  timed records four
+    // events per round (not under capture).
+    int enqueue_direct(bool timed) {
+        hipStream_t s = c->stream;
+        const size_t nd = P.desc.size();
+        const int log2w = log2_window(c);
+        const uint32_t grid = (uint32_t)c->num_cus * std::max<uint32_t>(c->tune.wg_per_cu, 1);
+        const uint32_t bgrid = (uint32_t)c->num_cus * 2;
+        HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, nd * sizeof(SegDesc),
+                              hipMemcpyHostToDevice, s));
+        if (ntask) {
+            HIPCHK(hipMemcpyAsync(B->d_btask.p, B->h_btask.p, ntask * sizeof(BucketTask),
+                                  hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(B->d_boff.p, B->h_boff.p, nboff * 8, hipMemcpyHostToDevice, s));
+            HIPCHK(launch_fill_u32(B->d_bcur.p, nbkt, 0, s));
+        }
+        HIPCHK(launch_fill_u32((uint32_t*)B->d_flags.p, 4, 0, s));
+        HIPCHK(launch_init_acc(B->d_acc.p, P.njobs, s));
+        for (size_t r = 0; r < rounds.size(); r++) {
+            const Round& R = rounds[r];
+            if (timed) HIPCHK(hipEventRecord(ev(4 * r), s));
+            if (!R.tasks.empty())
+                HIPCHK(launch_bucket_fill(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
+                                          c->d_primes.p, B->d_boff.p + R.boff_off,
+                                          B->d_bcur.p + R.bkt_off, B->d_bent.p,
+                                          (uint32_t*)(B->d_flags.p + 1), log2w, bgrid, s));
+            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 1), s));
+            const SieveArgs a = round_args(R);
+            HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, s));
+            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 2), s));
+            HIPCHK(launch_stitch(a, s));
+            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 3), s));
+        }
+        return enqueue_readback();
+    }
+
+    int enqueue_readback() {
+        hipStream_t s = c->stream;
+        if (copy_acc)
+            HIPCHK(hipMemcpyAsync(B->h_acc.p, B->d_acc.p, P.njobs * sizeof(JobAcc),
+                                  hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(B->h_rec.p, B->d_rec.p, P.desc.size() * sizeof(SegRec),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(B->h_flags.p, B->d_flags.p, 16, hipMemcpyDeviceToHost, s));
+        return VOSS_OK;
+    }
+
+    // Exact sequence: per round, a count pass, the host prefix sum, then the
+    // fill with exact offsets.  The rooms become the exact counts, so later
+    // direct runs of this plan cannot overflow.
+    int enqueue_exact(bool timed) {
+        hipStream_t s = c->stream;
+        const size_t nd = P.desc.size();
+        const int log2w = log2_window(c);
+        const uint32_t grid = (uint32_t)c->num_cus * std::max<uint32_t>(c->tune.wg_per_cu, 1);
+        const uint32_t bgrid = (uint32_t)c->num_cus * 2;
+        HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, nd * sizeof(SegDesc),
+                              hipMemcpyHostToDevice, s));
+        if (ntask)
+            HIPCHK(hipMemcpyAsync(B->d_btask.p, B->h_btask.p, ntask * sizeof(BucketTask),
+                                  hipMemcpyHostToDevice, s));
+        HIPCHK(launch_fill_u32((uint32_t*)B->d_flags.p, 4, 0, s));
+        HIPCHK(launch_init_acc(B->d_acc.p, P.njobs, s));
+        for (size_t r = 0; r < rounds.size(); r++) {
+            Round& R = rounds[r];
+            if (timed) HIPCHK(hipEventRecord(ev(4 * r), s));
+            if (!R.tasks.empty()) {
+                HIPCHK(launch_fill_u32(B->d_bcnt.p, R.nbuckets, 0, s));
+                HIPCHK(launch_bucket_count(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
+                                           c->d_primes.p, B->d_bcnt.p, log2w, bgrid, s));
+                HIPCHK(hipMemcpyAsync(B->h_bcnt.p, B->d_bcnt.p, R.nbuckets * 4,
+                                      hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                unsigned long long o = 0;
+                for (uint32_t b = 0; b < R.nbuckets; b++) {
+                    B->h_boff.p[R.boff_off + b] = o;
+                    R.room[b] = B->h_bcnt.p[b];
+                    o += R.room[b];
+                }
+                B->h_boff.p[R.boff_off + R.nbuckets] = o;
+                R.room_total = o;
+                if (o >= 0xFFFFFFFFull) return fail(VOSS_ENOMEM, "bucket round beyond 2^32 entries");
+                // the stream is idle here: growing the entry buffer is safe
+                HIPCHK(B->d_bent.reserve(std::max<uint64_t>(o, 1)));
+                HIPCHK(hipMemcpyAsync(B->d_boff.p + R.boff_off, B->h_boff.p + R.boff_off,
+                                      (R.nbuckets + 1) * 8, hipMemcpyHostToDevice, s));
+                HIPCHK(launch_fill_u32(B->d_bcur.p + R.bkt_off, R.nbuckets, 0, s));
+                HIPCHK(launch_bucket_fill(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
+                                          c->d_primes.p, B->d_boff.p + R.boff_off,
+                                          B->d_bcur.p + R.bkt_off, B->d_bent.p,
+                                          (uint32_t*)(B->d_flags.p + 1), log2w, bgrid, s));
+            }
+            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 1), s));
+            const SieveArgs a = round_args(R);
+            HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, s));
+            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 2), s));
+            HIPCHK(launch_stitch(a, s));
+            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 3), s));
+        }
+        return enqueue_readback();
+    }
+
+    int capture() {
+        drop_graph();
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_direct(false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIPCHK(e);
+        graph = g;
+        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        graph_primes = c->d_primes.p;
+        return VOSS_OK;
+    }
+
+    // Executes the plan until it completes without overflow.  On return
+    // B->h_acc, B->h_rec and B->h_large (nlarge records) hold the results.
+    int run(bool use_graph, ExecTimes* tm) {
+        ExecTimes t;
+        t.rounds = (uint32_t)rounds.size();
+        int rc;
+        if ((rc = ensure_events(4 * rounds.size()))) return rc;
+        if (gexec && graph_primes != c->d_primes.p) drop_graph();  // base primes moved
+        bool breakdown = false;
+        for (;;) {
+            if (!staged && (rc = stage())) return rc;
+            const bool timed = !use_graph;
+            HIPCHK(hipEventRecord(c->ev0, c->stream));
+            if (use_graph) {
+                if (!gexec && (rc = capture())) return rc;
+                HIPCHK(hipGraphLaunch(gexec, c->stream));
+            } else if ((rc = enqueue_direct(timed))) {
+                return rc;
+            }
+            HIPCHK(hipEventRecord(c->ev1, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            breakdown = timed;
+            if (B->h_flags.p[1]) {
+                // a bucket outgrew its room: exact counts, then direct again
+                t.reruns++;
+                HIPCHK(hipEventRecord(c->ev0, c->stream));
+                breakdown = true;
+                if ((rc = enqueue_exact(true))) return rc;
+                HIPCHK(hipEventRecord(c->ev1, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                staged = false;  // re-stage with the exact rooms next time
+                exact = true;
+                if (B->h_flags.p[1]) return fail(VOSS_EDEVICE, "bucket overflow with exact counts");
+            }
+            if (B->h_flags.p[0] > lcap) {
+                // keep every candidate (canonical top-K needs all of them)
+                t.reruns++;
+                lcap = B->h_flags.p[0] + (B->h_flags.p[0] >> 2);
+                staged = false;
+                continue;
+            }
+            break;
+        }
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        t.device_ms = ms;
+        if (!breakdown) {
+            t.sieve_ms = t.bucket_ms = -1;
+        } else {
+            for (size_t r = 0; r < rounds.size(); r++) {
+                float b = 0, sv = 0;
+                HIPCHK(hipEventElapsedTime(&b, ev(4 * r), ev(4 * r + 1)));
+                HIPCHK(hipEventElapsedTime(&sv, ev(4 * r + 1), ev(4 * r + 2)));
+                t.bucket_ms += b;
+                t.sieve_ms += sv;
+            }
+        }
+        for (const Round& R : rounds) t.bucket_entries += R.room_total;
+        nlarge = B->h_flags.p[0];
+        HIPCHK(B->h_large.reserve(std::max<uint64_t>(nlarge, 1)));
+        if (nlarge) {
+            HIPCHK(hipMemcpyAsync(B->h_large.p, B->d_large.p, nlarge * sizeof(LargeRaw),
                                   hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
         }
-        const LargeRaw* raw = c->h_large.p;
+        if (tm) *tm = t;
+        return VOSS_OK;
+    }
+
+    // Folds the results into per-job host accumulators: device sums, large
+    // records, and the pairs across round edges.
+    int collect(std::vector<JobHost>& jobs, std::vector<uint64_t>* counts) {
+        jobs.assign(P.njobs, JobHost());
         for (uint32_t j = 0; j < P.njobs; j++) {
-            if (acc[j].error) return fail(VOSS_ERANGE, "prime gap >= 2048 (beyond the device histogram)");
-            add_acc(jobs[j], acc[j]);
+            if (B->h_acc.p[j].error) return fail(VOSS_ERANGE, "prime gap >= 2048 (beyond the device histogram)");
+            add_acc(jobs[j], B->h_acc.p[j]);
         }
-        for (uint64_t k = 0; k < nkeep; k++) {
-            const LargeRaw& r = raw[k];
+        for (uint64_t k = 0; k < nlarge; k++) {
+            const LargeRaw& r = B->h_large.p[k];
             jobs[r.job].large.push_back({r.p_before + r.gap, (int32_t)r.gap,
                                          merit_of(r.p_before, r.gap)});
         }
+        const SegRec* rec = B->h_rec.p;
         if (counts) {
-            counts->resize(off + nd);
-            for (uint32_t i = 0; i < nd; i++) (*counts)[off + i] = rec[i].count;
+            counts->resize(P.desc.size());
+            for (size_t i = 0; i < P.desc.size(); i++) (*counts)[i] = rec[i].count;
         }
-        // launch boundary: pair the previous launch's last prime of a job
-        // with this launch's first non-empty window of the same job
-        std::vector<uint8_t> seen(P.njobs, 0);
-        for (uint32_t i = 0; i < nd; i++) {
-            const SegDesc& d = P.desc[off + i];
-            if (rec[i].count == 0) continue;
-            if (!seen[d.job]) {
-                seen[d.job] = 1;
-                if (carry_has[d.job])
-                    host_pair(jobs[d.job], carry_last[d.job], rec[i].first,
-                              carry_lseg[d.job] == d.lseg);
+        // round edge: pair the last prime of a job before the round with the
+        // first non-empty window of the same job in the round
+        std::vector<uint64_t> carry_last(P.njobs, 0);
+        std::vector<uint32_t> carry_lseg(P.njobs, 0);
+        std::vector<uint8_t> carry_has(P.njobs, 0), seen(P.njobs, 0);
+        for (const Round& R : rounds) {
+            std::fill(seen.begin(), seen.end(), 0);
+            for (size_t i = R.off; i < R.off + R.nd; i++) {
+                const SegDesc& d = P.desc[i];
+                if (rec[i].count == 0) continue;
+                if (!seen[d.job]) {
+                    seen[d.job] = 1;
+                    if (carry_has[d.job])
+                        host_pair(jobs[d.job], carry_last[d.job], rec[i].first,
+                                  carry_lseg[d.job] == d.lseg);
+                }
+                carry_has[d.job] = 1;
+                carry_last[d.job] = rec[i].last;
+                carry_lseg[d.job] = d.lseg;
             }
-            carry_has[d.job] = 1;
-            carry_last[d.job] = rec[i].last;
-            carry_lseg[d.job] = d.lseg;
         }
+        return VOSS_OK;
     }
-    if (times) {
-        times->device_ms = total_ms + tmb.bucket_ms;
-        times->sieve_ms = sieve_ms;
-        times->bucket_ms = tmb.bucket_ms;
-        times->bucket_entries = tmb.bucket_entries;
-        times->rounds = (uint32_t)rounds.size();
+};
+
+// One-shot execution with the context's buffers.
+int execute(voss_ctx* c, Plan&& P, std::vector<JobHost>& jobs, ExecTimes* times,
+            std::vector<uint64_t>* counts = nullptr, bool use_graph = false) {
+    jobs.assign(P.njobs, JobHost());
+    if (P.njobs == 0 || P.desc.empty()) {
+        if (times) *times = ExecTimes();
+        return VOSS_OK;
     }
-    return VOSS_OK;
+    Executor X(c, &c->eb);
+    X.P = std::move(P);
+    X.plan_rounds();
+    int rc = X.run(use_graph && c->graph_mode, times);
+    if (rc) return rc;
+    return X.collect(jobs, counts);
 }
 
 double now_s() {
@@ -874,25 +1136,11 @@ void voss_destroy(voss_ctx* c) {
     c->d_primes.release();
     c->d_pat.release();
     c->d_lg_thresh.release();
-    c->d_desc.release();
-    c->d_rec.release();
-    c->d_acc.release();
-    c->d_large.release();
-    c->d_large_n.release();
-    c->d_btask.release();
-    c->d_bcnt.release();
-    c->d_bcur.release();
-    c->d_boff.release();
-    c->d_bent.release();
-    c->h_bcnt.release();
-    c->h_boff.release();
+    c->eb.release();
+    for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+    c->evs.clear();
     c->d_emit.release();
     c->d_emit_off.release();
-    c->h_desc.release();
-    c->h_rec.release();
-    c->h_acc.release();
-    c->h_large.release();
-    c->h_large_n.release();
     c->d_mr_n.release();
     c->d_mr_v.release();
     c->d_surv.release();
@@ -930,7 +1178,88 @@ int voss_set_tuning(voss_ctx* c, const voss_tuning* t) {
         n.bucket_limit = t->bucket_limit;
     }
     if (t->bucket_min_windows) n.bucket_min_windows = t->bucket_min_windows;
+    if (t->bucket_room_pct) {
+        if (t->bucket_room_pct > 1000) return fail(VOSS_EINVAL, "bucket_room_pct beyond 1000");
+        n.bucket_room_pct = t->bucket_room_pct;
+    }
     c->tune = n;
+    return VOSS_OK;
+}
+
+int voss_autotune(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
+                  size_t max_intervals, int repeats, voss_tuning* best, double* best_seconds) {
+    if (!c || (n && (!lo || !hi))) return fail(VOSS_EINVAL, "NULL argument");
+    if (n == 0) return fail(VOSS_EINVAL, "no intervals to tune on");
+    if (repeats < 1) repeats = 1;
+    if (max_intervals == 0) max_intervals = n;
+    // evenly spread sample of the workload
+    std::vector<uint64_t> slo, shi;
+    const size_t m = std::min(n, max_intervals);
+    for (size_t k = 0; k < m; k++) {
+        const size_t i = k * n / m;
+        slo.push_back(lo[i]);
+        shi.push_back(hi[i]);
+    }
+    std::vector<voss_interval_result> res(m);
+    auto time_it = [&](const voss_tuning& t, double* sec) -> int {
+        int rc = voss_set_tuning(c, &t);
+        if (rc) return rc;
+        double bestt = 1e300;
+        for (int r = 0; r <= repeats; r++) {
+            for (auto& x : res) { x.large = nullptr; x.large_capacity = 0; }
+            const double t0 = now_s();
+            if ((rc = voss_run_batch(c, slo.data(), shi.data(), m, res.data(), nullptr))) return rc;
+            const double dt = now_s() - t0;
+            if (r > 0) bestt = std::min(bestt, dt);  // the first run warms up
+        }
+        *sec = bestt;
+        return VOSS_OK;
+    };
+    voss_tuning cur = c->tune;
+    double cur_s = 0;
+    int rc = time_it(cur, &cur_s);
+    if (rc) return rc;
+    // (window bytes, workgroups per CU): the LDS of one CU holds one 128 KiB
+    // window, two 64 KiB or four 32 KiB ones
+    const std::vector<std::pair<uint32_t, uint32_t>> shapes = {
+        {131072, 1}, {65536, 2}, {65536, 1}, {32768, 4}};
+    const std::vector<uint32_t> tiers = {512, 1024, 2048};
+    const std::vector<uint32_t> blimits = {1u << 21, 1u << 23, 1u << 25, 0xFFFFFFFFu};
+    const std::vector<uint32_t> bmins = {2, 4, 8};
+    for (int pass = 0; pass < 2; pass++) {
+        bool moved = false;
+        for (int dim = 0; dim < 4; dim++) {
+            const size_t nv = dim == 0 ? shapes.size() : dim == 1 ? tiers.size()
+                            : dim == 2 ? blimits.size() : bmins.size();
+            for (size_t k = 0; k < nv; k++) {
+                voss_tuning t = cur;
+                if (dim == 0) {
+                    t.window_bytes = shapes[k].first;
+                    t.wg_per_cu = shapes[k].second;
+                    if (t.bucket_limit < t.window_bytes) t.bucket_limit = t.window_bytes;
+                } else if (dim == 1) {
+                    t.tier_a_limit = tiers[k];
+                } else if (dim == 2) {
+                    if (blimits[k] < t.window_bytes) continue;
+                    t.bucket_limit = blimits[k];
+                } else {
+                    t.bucket_min_windows = bmins[k];
+                }
+                if (std::memcmp(&t, &cur, sizeof t) == 0) continue;
+                double sec = 0;
+                if ((rc = time_it(t, &sec))) return rc;
+                if (sec < cur_s * 0.98) {  // ignore changes within the noise
+                    cur = t;
+                    cur_s = sec;
+                    moved = true;
+                }
+            }
+        }
+        if (!moved) break;
+    }
+    if ((rc = voss_set_tuning(c, &cur))) return rc;
+    if (best) *best = cur;
+    if (best_seconds) *best_seconds = cur_s;
     return VOSS_OK;
 }
 
@@ -974,11 +1303,12 @@ int plain_range(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t s0, uint64_t
     const uint32_t W = c->tune.window_bytes;
     int rc = ensure_primes(c, (N / 30 / W + 1) * W * 30);
     if (rc) return rc;
-    const Plan P = plain_plan(c, N, seg_num, s0, s1);
+    Plan P = plain_plan(c, N, seg_num, s0, s1);
+    *nwin = P.desc.size();
     std::vector<JobHost> jobs;
-    const uint64_t cap = out->large ? out->large_capacity : 0;
     std::vector<uint64_t> counts;
-    if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm, &counts))) return rc;
+    (void)out;
+    if ((rc = execute(c, std::move(P), jobs, &tm, &counts))) return rc;
     if (s0 == 0 && s1 == (N + seg_num - 1) / seg_num) {
         c->last_N = N;
         c->last_seg = seg_num;
@@ -986,7 +1316,77 @@ int plain_range(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t s0, uint64_t
         c->last_counts.swap(counts);
     }
     J = std::move(jobs[0]);
-    *nwin = P.desc.size();
+    return VOSS_OK;
+}
+
+// Checks a batch and sizes the base primes for it.
+int batch_args(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    if (n && (!lo || !hi)) return fail(VOSS_EINVAL, "NULL argument");
+    if (n > 0xFFFFFFFFull) return fail(VOSS_EINVAL, "too many intervals");
+    uint64_t maxhi = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (hi[i] < lo[i]) return fail(VOSS_EINVAL, "hi < lo");
+        if (hi[i] > (1ull << 62)) return fail(VOSS_EINVAL, "hi beyond 2^62");
+        maxhi = std::max(maxhi, hi[i]);
+    }
+    int rc = set_device(c);
+    if (rc) return rc;
+    return ensure_primes(c, maxhi / 30 * 30 + 30ull * c->tune.window_bytes + 30);
+}
+
+int check_interval_out(const voss_interval_result* out, size_t n) {
+    if (n && !out) return fail(VOSS_EINVAL, "NULL argument");
+    for (size_t i = 0; i < n; i++)
+        if (out[i].large_capacity && !out[i].large) return fail(VOSS_EINVAL, "large buffer is NULL");
+    return VOSS_OK;
+}
+
+// Job i = interval i, each on its own window grid.
+Plan plan_batch(const voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n) {
+    Plan P;
+    P.njobs = (uint32_t)n;
+    const uint32_t W = c->tune.window_bytes;
+    for (size_t i = 0; i < n; i++) {
+        if (hi[i] <= 7) continue;
+        const uint64_t a = std::max<uint64_t>(lo[i], 7), b = hi[i] - 1;
+        plan_range(c, P, a, b, (uint32_t)i, 0, W, (a / 30) & ~4095ull);
+    }
+    return P;
+}
+
+void batch_stats(voss_result* stats, const ExecTimes& tm, double t0, uint64_t nwin) {
+    if (!stats) return;
+    clear_result(stats);
+    stats->seconds = now_s() - t0;
+    stats->device_ms = tm.device_ms;
+    stats->sieve_ms = tm.sieve_ms;
+    stats->windows = nwin;
+    stats->bucket_ms = tm.bucket_ms;
+    stats->bucket_entries = tm.bucket_entries;
+}
+
+int finish_batch(std::vector<JobHost>& jobs, const uint64_t* lo, const uint64_t* hi, size_t n,
+                 voss_interval_result* out) {
+    for (size_t i = 0; i < n; i++) {
+        finish_interval(jobs[i], lo[i], hi[i], &out[i]);
+        if (jobs[i].overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");
+    }
+    return VOSS_OK;
+}
+
+int run_batch(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
+              voss_interval_result* out, voss_result* stats, bool use_graph) {
+    int rc = batch_args(c, lo, hi, n);
+    if (rc || (rc = check_interval_out(out, n))) return rc;
+    const double t0 = now_s();
+    Plan P = plan_batch(c, lo, hi, n);
+    const uint64_t nwin = P.desc.size();
+    std::vector<JobHost> jobs;
+    ExecTimes tm;
+    if ((rc = execute(c, std::move(P), jobs, &tm, nullptr, use_graph))) return rc;
+    if ((rc = finish_batch(jobs, lo, hi, n, out))) return rc;
+    batch_stats(stats, tm, t0, nwin);
     return VOSS_OK;
 }
 
@@ -1073,14 +1473,14 @@ int voss_run_interval(voss_ctx* c, uint64_t lo, uint64_t hi, voss_result* out) {
     }
     std::vector<JobHost> jobs;
     ExecTimes tm;
-    const uint64_t cap = out->large ? out->large_capacity : 0;
-    if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm))) return rc;
+    const uint64_t nwin = P.desc.size();
+    if ((rc = execute(c, std::move(P), jobs, &tm))) return rc;
     finish_interval(jobs[0], lo, hi, out);
     if (jobs[0].overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");
     out->seconds = now_s() - t0;
     out->device_ms = tm.device_ms;
     out->sieve_ms = tm.sieve_ms;
-    out->windows = P.desc.size();
+    out->windows = nwin;
     out->bucket_ms = tm.bucket_ms;
     out->bucket_entries = tm.bucket_entries;
     return VOSS_OK;
@@ -1088,46 +1488,7 @@ int voss_run_interval(voss_ctx* c, uint64_t lo, uint64_t hi, voss_result* out) {
 
 int voss_run_batch(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
                    voss_interval_result* out, voss_result* stats) {
-    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
-    if (n && (!lo || !hi || !out)) return fail(VOSS_EINVAL, "NULL argument");
-    if (n > 0xFFFFFFFFull) return fail(VOSS_EINVAL, "too many intervals");
-    int rc;
-    uint64_t maxhi = 0, cap = 0;
-    for (size_t i = 0; i < n; i++) {
-        if (out[i].large_capacity && !out[i].large) return fail(VOSS_EINVAL, "large buffer is NULL");
-        if (hi[i] < lo[i]) return fail(VOSS_EINVAL, "hi < lo");
-        if (hi[i] > (1ull << 62)) return fail(VOSS_EINVAL, "hi beyond 2^62");
-        maxhi = std::max(maxhi, hi[i]);
-        if (out[i].large) cap += out[i].large_capacity;
-    }
-    if ((rc = set_device(c))) return rc;
-    const uint32_t W = c->tune.window_bytes;
-    if ((rc = ensure_primes(c, maxhi / 30 * 30 + 30ull * W + 30))) return rc;
-    const double t0 = now_s();
-    Plan P;
-    P.njobs = (uint32_t)n;
-    for (size_t i = 0; i < n; i++) {
-        if (hi[i] <= 7) continue;
-        const uint64_t a = std::max<uint64_t>(lo[i], 7), b = hi[i] - 1;
-        plan_range(c, P, a, b, (uint32_t)i, 0, W, (a / 30) & ~4095ull);
-    }
-    std::vector<JobHost> jobs;
-    ExecTimes tm;
-    if ((rc = execute(c, P, jobs, std::max<uint64_t>(cap, 1), &tm))) return rc;
-    for (size_t i = 0; i < n; i++) {
-        finish_interval(jobs[i], lo[i], hi[i], &out[i]);
-        if (jobs[i].overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");
-    }
-    if (stats) {
-        clear_result(stats);
-        stats->seconds = now_s() - t0;
-        stats->device_ms = tm.device_ms;
-        stats->sieve_ms = tm.sieve_ms;
-        stats->windows = P.desc.size();
-        stats->bucket_ms = tm.bucket_ms;
-        stats->bucket_entries = tm.bucket_entries;
-    }
-    return VOSS_OK;
+    return run_batch(c, lo, hi, n, out, stats, c ? c->graph_mode != 0 : false);
 }
 
 int voss_run_batch_plain(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
@@ -1139,7 +1500,7 @@ int voss_run_batch_plain(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, si
     uint64_t win = 0;
     for (size_t i = 0; i < n; i++) {
         voss_result st;
-        int rc = voss_run_batch(c, lo + i, hi + i, 1, out + i, &st);
+        int rc = run_batch(c, lo + i, hi + i, 1, out + i, &st, false);
         if (rc) return rc;
         dev += st.device_ms;
         sv += st.sieve_ms;
@@ -1152,6 +1513,76 @@ int voss_run_batch_plain(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, si
         stats->sieve_ms = sv;
         stats->windows = win;
     }
+    return VOSS_OK;
+}
+
+}  // extern "C"
+
+struct voss_batch {
+    voss_ctx* c = nullptr;
+    voss_tuning tune{};
+    std::vector<uint64_t> lo, hi;
+    uint64_t nwin = 0;
+    ExecBufs B;
+    std::unique_ptr<Executor> X;
+};
+
+extern "C" {
+
+int voss_batch_prepare(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
+                       voss_batch** out) {
+    if (!out) return fail(VOSS_EINVAL, "out is NULL");
+    *out = nullptr;
+    int rc = batch_args(c, lo, hi, n);
+    if (rc) return rc;
+    std::unique_ptr<voss_batch> b(new voss_batch());
+    b->c = c;
+    b->tune = c->tune;
+    b->lo.assign(lo, lo + n);
+    b->hi.assign(hi, hi + n);
+    b->X.reset(new Executor(c, &b->B));
+    b->X->P = plan_batch(c, lo, hi, n);
+    b->nwin = b->X->P.desc.size();
+    b->X->plan_rounds();
+    *out = b.release();
+    return VOSS_OK;
+}
+
+int voss_batch_run(voss_batch* b, voss_interval_result* out, voss_result* stats) {
+    if (!b) return fail(VOSS_EINVAL, "batch is NULL");
+    voss_ctx* c = b->c;
+    const size_t n = b->lo.size();
+    int rc = check_interval_out(out, n);
+    if (rc) return rc;
+    if (std::memcmp(&c->tune, &b->tune, sizeof(voss_tuning)) != 0)
+        return fail(VOSS_EINVAL, "tuning changed since voss_batch_prepare");
+    if ((rc = set_device(c))) return rc;
+    const double t0 = now_s();
+    std::vector<JobHost> jobs;
+    ExecTimes tm;
+    if (b->X->P.desc.empty()) {
+        jobs.assign(n, JobHost());
+    } else {
+        if ((rc = b->X->run(true, &tm))) return rc;
+        if ((rc = b->X->collect(jobs, nullptr))) return rc;
+    }
+    if ((rc = finish_batch(jobs, b->lo.data(), b->hi.data(), n, out))) return rc;
+    batch_stats(stats, tm, t0, b->nwin);
+    return VOSS_OK;
+}
+
+void voss_batch_free(voss_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->c->device);
+    (void)hipStreamSynchronize(b->c->stream);
+    b->X.reset();
+    b->B.release();
+    delete b;
+}
+
+int voss_set_graph_mode(voss_ctx* c, int enable) {
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    c->graph_mode = enable ? 1 : 0;
     return VOSS_OK;
 }
 
@@ -1212,19 +1643,20 @@ int voss_sample_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, int32_t* gap_se
     uint64_t rows = 0;
     if (!emit_desc.empty()) {
         const uint32_t nd = (uint32_t)emit_desc.size();
-        HIPCHK(c->d_desc.reserve(nd));
-        HIPCHK(c->d_rec.reserve(nd));
+        ExecBufs& B = c->eb;
+        HIPCHK(B.d_desc.reserve(nd));
+        HIPCHK(B.d_rec.reserve(nd));
         HIPCHK(c->d_emit_off.reserve(nd));
         HIPCHK(c->d_emit.reserve(std::max<unsigned long long>(total, 1)));
-        HIPCHK(c->d_acc.reserve(1));
-        HIPCHK(c->d_large.reserve(1));
-        HIPCHK(c->d_large_n.reserve(1));
-        HIPCHK(hipMemcpyAsync(c->d_desc.p, emit_desc.data(), nd * sizeof(SegDesc),
+        HIPCHK(B.d_acc.reserve(1));
+        HIPCHK(B.d_large.reserve(1));
+        HIPCHK(B.d_flags.reserve(2));
+        HIPCHK(hipMemcpyAsync(B.d_desc.p, emit_desc.data(), nd * sizeof(SegDesc),
                               hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->d_emit_off.p, emit_off.data(), nd * 8, hipMemcpyHostToDevice,
                               c->stream));
-        HIPCHK(launch_init_acc(c->d_acc.p, 1, c->stream));
-        SieveArgs a = make_args(c, nd, 1);
+        HIPCHK(launch_init_acc(B.d_acc.p, 1, c->stream));
+        SieveArgs a = make_args(c, B, nd, 1);
         HIPCHK(launch_sieve_emit(a, (int)W, (uint32_t)c->num_cus * c->tune.wg_per_cu, c->stream));
         std::vector<unsigned long long> primes(total);
         HIPCHK(hipMemcpyAsync(primes.data(), c->d_emit.p, total * 8, hipMemcpyDeviceToHost,
@@ -1429,72 +1861,42 @@ uint64_t req_hbm(const voss_server* s, const ServeReq& r) {
 // requests on the host.
 int serve_sieve_step(voss_server* s, const std::vector<std::pair<ReqPtr, size_t>>& take) {
     voss_ctx* c = s->c;
-    Plan P;
-    P.njobs = s->cfg.max_active;
+    Executor X(c, &c->eb);
+    X.P.njobs = s->cfg.max_active;
+    X.copy_acc = false;
     std::vector<ServeReq*> by_slot(s->cfg.max_active, nullptr);
     for (const auto& t : take) {
         ServeReq* r = t.first.get();
         by_slot[r->slot] = r;
-        for (size_t k = 0; k < t.second; k++) P.desc.push_back(r->win[r->next + k]);
+        for (size_t k = 0; k < t.second; k++) X.P.desc.push_back(r->win[r->next + k]);
     }
-    if (P.desc.empty()) return VOSS_OK;
-    std::vector<Round> rounds;
-    make_rounds(c, P, rounds);
-    uint64_t lcap = std::max<uint64_t>(c->d_large.cap, 1u << 20);
-    HIPCHK(c->d_large.reserve(lcap));
-    HIPCHK(c->d_large_n.reserve(1));
-    HIPCHK(c->h_large_n.reserve(1));
-    const uint32_t grid = (uint32_t)c->num_cus * std::max<uint32_t>(c->tune.wg_per_cu, 1);
-    ExecTimes tmb;
-    for (size_t ri = 0; ri < rounds.size(); ri++) {
-        const Round& R = rounds[ri];
-        const uint32_t nd = (uint32_t)R.nd;
-        int rc = run_buckets(c, R, &tmb);
-        if (rc) return rc;
-        HIPCHK(c->d_desc.reserve(nd));
-        HIPCHK(c->d_rec.reserve(nd));
-        HIPCHK(c->h_desc.reserve(nd));
-        HIPCHK(c->h_rec.reserve(nd));
-        std::memcpy(c->h_desc.p, R.desc.data(), nd * sizeof(SegDesc));
-        HIPCHK(hipMemcpyAsync(c->d_desc.p, c->h_desc.p, nd * sizeof(SegDesc),
-                              hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipMemsetAsync(c->d_large_n.p, 0, sizeof(unsigned long long), c->stream));
-        SieveArgs a = make_args(c, nd, lcap);
-        a.acc = s->d_acc.p;  // resident per-request slots
-        HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, c->stream));
-        HIPCHK(launch_stitch(a, c->stream));
-        HIPCHK(hipMemcpyAsync(c->h_rec.p, c->d_rec.p, nd * sizeof(SegRec), hipMemcpyDeviceToHost,
-                              c->stream));
-        HIPCHK(hipMemcpyAsync(c->h_large_n.p, c->d_large_n.p, 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        if (c->h_large_n.p[0] > lcap) {
-            // the slots already hold this round's sums: this cannot be re-run
-            return fail(VOSS_ENOMEM, "large-gap list overflow in a serving step");
-        }
-        const uint64_t nl = c->h_large_n.p[0];
-        HIPCHK(c->h_large.reserve(nl));
-        if (nl) {
-            HIPCHK(hipMemcpy(c->h_large.p, c->d_large.p, nl * sizeof(LargeRaw),
-                             hipMemcpyDeviceToHost));
-        }
-        for (uint64_t k = 0; k < nl; k++) {
-            const LargeRaw& x = c->h_large.p[k];
-            ServeReq* r = by_slot[x.job];
-            r->J.large.push_back({x.p_before + x.gap, (int32_t)x.gap, merit_of(x.p_before, x.gap)});
-        }
-        // edges between this round and what the request saw before
-        std::vector<uint8_t> seen(s->cfg.max_active, 0);
-        for (uint32_t i = 0; i < nd; i++) {
-            const SegDesc& d = R.desc[i];
-            const SegRec& rc2 = c->h_rec.p[i];
-            if (rc2.count == 0) continue;
+    if (X.P.desc.empty()) return VOSS_OK;
+    X.plan_rounds();
+    int rc = X.run(false, nullptr);
+    if (rc) return rc;
+    // the step's sums into the requests' resident slots (stream order puts
+    // this before any later read of a slot)
+    HIPCHK(launch_add_acc(s->d_acc.p, c->eb.d_acc.p, s->cfg.max_active, c->stream));
+    for (uint64_t k = 0; k < X.nlarge; k++) {
+        const LargeRaw& x = c->eb.h_large.p[k];
+        ServeReq* r = by_slot[x.job];
+        r->J.large.push_back({x.p_before + x.gap, (int32_t)x.gap, merit_of(x.p_before, x.gap)});
+    }
+    // edges between each round and what the request saw before it
+    const SegRec* rec = c->eb.h_rec.p;
+    std::vector<uint8_t> seen(s->cfg.max_active, 0);
+    for (const Round& R : X.rounds) {
+        std::fill(seen.begin(), seen.end(), 0);
+        for (size_t i = R.off; i < R.off + R.nd; i++) {
+            const SegDesc& d = X.P.desc[i];
+            if (rec[i].count == 0) continue;
             ServeReq* r = by_slot[d.job];
             if (!seen[d.job]) {
                 seen[d.job] = 1;
-                if (r->carry_has) host_pair(r->J, r->carry_last, rc2.first, true);
+                if (r->carry_has) host_pair(r->J, r->carry_last, rec[i].first, true);
             }
             r->carry_has = true;
-            r->carry_last = rc2.last;
+            r->carry_last = rec[i].last;
         }
     }
     return VOSS_OK;

@@ -237,7 +237,8 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             if (D.bucket != kNoBucket) {
                 // large primes: their hits in this window, precomputed
                 const unsigned long long e0 = a.bucket_off[D.bucket];
-                const uint32_t ne = a.bucket_cnt[D.bucket];
+                const unsigned long long cap = a.bucket_off[D.bucket + 1] - e0;
+                const uint32_t ne = (uint32_t)min((unsigned long long)a.bucket_cnt[D.bucket], cap);
                 for (uint32_t k = tid; k < ne; k += BLOCK) {
                     const uint32_t e = a.bucket_ent[e0 + k];
                     const uint32_t b = e >> 3;
@@ -486,12 +487,75 @@ __global__ void init_acc_kernel(JobAcc* acc, uint32_t njobs) {
     if (threadIdx.x == 0) a->first = ~0ull;
 }
 
-// Large-prime buckets.  One workgroup per task; thread k handles prime
-// i0 + k/8, residue class k%8, and walks its hits across the task's
-// windows (at most a few per window for primes beyond the window length).
-// COUNT adds the hits per window into bucket_cnt; FILL recounts them in
-// LDS, reserves each window's slots with one global atomic, and writes
-// the entries.  Both passes walk the same hits in the same order.
+// Large-prime buckets.  One workgroup per task (a tile of consecutive
+// windows of one job and a range of primes).  Two ways to enumerate the
+// composite multiples m*p (m >= 2, coprime to 30) of a prime in the tile:
+//   class mode  thread per (prime, residue class): first multiple of the
+//               class by Barrett reduction, then stride p bytes.  Best when
+//               p is below the tile length (several hits per class);
+//   walk mode   thread per prime: one Barrett reduction gives the first
+//               multiple at or after the tile start, then the thread steps
+//               m -> m+1 (p/30 bytes and p%30 in the residue) and keeps the
+//               multiples coprime to 30.  Best for p beyond the tile length,
+//               where a class mode thread mostly finds nothing.
+// COUNT adds the hits per window into bucket_cnt.  FILL counts them in
+// LDS, reserves each window's slots with one global atomic on its cursor,
+// and writes the entries (offset << 3) | bit.  A bucket holds
+// bucket_off[b+1] - bucket_off[b] entries: a reservation beyond that
+// writes nothing and sets *overflow (the caller then re-runs with exact
+// counts), so no write ever leaves the bucket.
+
+// wheel bit of residue r (r coprime to 30), four bits per residue
+constexpr unsigned long long res_nibbles(int half) {
+    unsigned long long v = 0;
+    for (int o = 0; o < 8; o++) {
+        const int r = (int)((kW30Bytes >> (8 * o)) & 0xFF);
+        if ((r >> 4) == half) v |= (unsigned long long)o << (4 * (r & 15));
+    }
+    return v;
+}
+constexpr unsigned long long kResLo = res_nibbles(0), kResHi = res_nibbles(1);
+constexpr uint32_t kCoprime30 = (1u << 1) | (1u << 7) | (1u << 11) | (1u << 13) | (1u << 17) |
+                                (1u << 19) | (1u << 23) | (1u << 29);
+__device__ __forceinline__ uint32_t wheel_bit_of_residue(uint32_t r) {
+    return (uint32_t)(((r < 16 ? kResLo : kResHi) >> (4u * (r & 15u))) & 7u);
+}
+
+// Calls f(byte offset in the tile, wheel bit) for every hit of work item k.
+template <typename F>
+__device__ __forceinline__ void bucket_hits(const BucketTask& T, const PrimeRec* __restrict__ primes,
+                                            uint32_t k, uint32_t span, F&& f) {
+    if (T.walk) {
+        const PrimeRec pr = primes[T.i0 + k];
+        const uint32_t p = pr.p;
+        const uint64_t X = 30ull * T.w0;  // first number of the tile
+        uint32_t x;                       // first multiple m*p >= X with m >= 2, minus X
+        if (X < 2ull * p) {
+            x = (uint32_t)(2ull * p - X);
+        } else {
+            const uint32_t r = mod_u64(X, p, pr.magic);
+            x = r ? p - r : 0u;
+        }
+        uint32_t b = x / 30u, rr = x - 30u * b;
+        const uint32_t pd = p / 30u, pm = p - 30u * pd;
+        while (b < span) {
+            if ((kCoprime30 >> rr) & 1u) f(b, wheel_bit_of_residue(rr));
+            b += pd;
+            rr += pm;
+            if (rr >= 30u) {
+                rr -= 30u;
+                b++;
+            }
+        }
+    } else {
+        const PrimeRec pr = primes[T.i0 + (k >> 3)];
+        const int j = (int)(k & 7u);
+        const uint32_t r = mod_u64(T.w0, pr.p, pr.magic);
+        const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
+        for (uint32_t b = first_pos(pr.p, r, j, T.w0); b < span; b += pr.p) f(b, bit);
+    }
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restrict__ tasks,
                                                       uint32_t ntask,
@@ -499,23 +563,21 @@ __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restri
                                                       uint32_t* __restrict__ bucket_cnt,
                                                       const unsigned long long* __restrict__ bucket_off,
                                                       uint32_t* __restrict__ bucket_cursor,
-                                                      uint32_t* __restrict__ entries, int log2w) {
+                                                      uint32_t* __restrict__ entries,
+                                                      uint32_t* __restrict__ overflow, int log2w) {
     __shared__ uint32_t cnt[kBucketTileMax];
+    __shared__ uint32_t lim[FILL ? kBucketTileMax : 1];
     const uint32_t tid = threadIdx.x;
     const uint32_t wmask = (1u << log2w) - 1u;
     for (uint32_t t = blockIdx.x; t < ntask; t += gridDim.x) {
         const BucketTask T = tasks[t];
-        const uint64_t span = (uint64_t)T.nwin << log2w;
+        const uint32_t span = T.nwin << log2w;  // bytes; at most 2^30
         for (uint32_t i = tid; i < T.nwin; i += blockDim.x) cnt[i] = 0;
         __syncthreads();
-        const uint32_t nthr = (T.i1 - T.i0) * 8u;
-        for (uint32_t k = tid; k < nthr; k += blockDim.x) {
-            const PrimeRec pr = primes[T.i0 + (k >> 3)];
-            const int j = (int)(k & 7u);
-            const uint32_t r = mod_u64(T.w0, pr.p, pr.magic);
-            for (uint64_t b = first_pos(pr.p, r, j, T.w0); b < span; b += pr.p)
-                atomicAdd(&cnt[(uint32_t)(b >> log2w)], 1u);
-        }
+        const uint32_t nthr = (T.i1 - T.i0) * (T.walk ? 1u : 8u);
+        for (uint32_t k = tid; k < nthr; k += blockDim.x)
+            bucket_hits(T, primes, k, span,
+                        [&](uint32_t b, uint32_t) { atomicAdd(&cnt[b >> log2w], 1u); });
         __syncthreads();
         if (!FILL) {
             for (uint32_t i = tid; i < T.nwin; i += blockDim.x)
@@ -523,21 +585,40 @@ __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restri
             __syncthreads();
             continue;
         }
-        for (uint32_t i = tid; i < T.nwin; i += blockDim.x)
-            if (cnt[i])
-                cnt[i] = (uint32_t)bucket_off[T.b0 + i] + atomicAdd(&bucket_cursor[T.b0 + i], cnt[i]);
-        __syncthreads();
-        for (uint32_t k = tid; k < nthr; k += blockDim.x) {
-            const PrimeRec pr = primes[T.i0 + (k >> 3)];
-            const int j = (int)(k & 7u);
-            const uint32_t r = mod_u64(T.w0, pr.p, pr.magic);
-            const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
-            for (uint64_t b = first_pos(pr.p, r, j, T.w0); b < span; b += pr.p) {
-                const uint32_t slot = atomicAdd(&cnt[(uint32_t)(b >> log2w)], 1u);
-                entries[slot] = (((uint32_t)b & wmask) << 3) | bit;
-            }
+        for (uint32_t i = tid; i < T.nwin; i += blockDim.x) {
+            if (!cnt[i]) continue;
+            const unsigned long long o0 = bucket_off[T.b0 + i], o1 = bucket_off[T.b0 + i + 1];
+            const uint32_t start = atomicAdd(&bucket_cursor[T.b0 + i], cnt[i]);
+            if ((unsigned long long)start + cnt[i] > o1 - o0) atomicOr(overflow, 1u);
+            cnt[i] = (uint32_t)o0 + start;
+            lim[i] = (uint32_t)o1;
         }
         __syncthreads();
+        for (uint32_t k = tid; k < nthr; k += blockDim.x)
+            bucket_hits(T, primes, k, span, [&](uint32_t b, uint32_t bit) {
+                const uint32_t w = b >> log2w;
+                const uint32_t slot = atomicAdd(&cnt[w], 1u);
+                if (slot < lim[w]) entries[slot] = ((b & wmask) << 3) | bit;
+            });
+        __syncthreads();
+    }
+}
+
+// dst[j] += src[j] for the accumulators of n jobs: sums, min of first, max
+// of last, or of the error bits.  Used to fold a step's sums into
+// long-lived per-request accumulators.
+__global__ void add_acc_kernel(JobAcc* __restrict__ dst, const JobAcc* __restrict__ src) {
+    JobAcc& d = dst[blockIdx.x];
+    const JobAcc& s = src[blockIdx.x];
+    constexpr uint32_t kSums = offsetof(JobAcc, first) / sizeof(unsigned long long);
+    unsigned long long* dp = reinterpret_cast<unsigned long long*>(&d);
+    const unsigned long long* sp = reinterpret_cast<const unsigned long long*>(&s);
+    for (uint32_t i = threadIdx.x; i < kSums; i += blockDim.x) dp[i] += sp[i];
+    if (threadIdx.x == 0) {
+        d.first = min(d.first, s.first);
+        d.last = max(d.last, s.last);
+        d.large_count += s.large_count;
+        d.error |= s.error;
     }
 }
 
@@ -641,17 +722,24 @@ hipError_t launch_bucket_count(const BucketTask* tasks, uint32_t ntask, const Pr
     if (ntask == 0) return hipSuccess;
     if (grid > ntask) grid = ntask;
     hipLaunchKernelGGL(bucket_kernel<false>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes,
-                       bucket_cnt, nullptr, nullptr, nullptr, log2w);
+                       bucket_cnt, nullptr, nullptr, nullptr, nullptr, log2w);
     return hipGetLastError();
 }
 
 hipError_t launch_bucket_fill(const BucketTask* tasks, uint32_t ntask, const PrimeRec* primes,
                               const unsigned long long* bucket_off, uint32_t* bucket_cursor,
-                              uint32_t* entries, int log2w, uint32_t grid, hipStream_t s) {
+                              uint32_t* entries, uint32_t* overflow, int log2w, uint32_t grid,
+                              hipStream_t s) {
     if (ntask == 0) return hipSuccess;
     if (grid > ntask) grid = ntask;
     hipLaunchKernelGGL(bucket_kernel<true>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes,
-                       nullptr, bucket_off, bucket_cursor, entries, log2w);
+                       nullptr, bucket_off, bucket_cursor, entries, overflow, log2w);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_acc(JobAcc* dst, const JobAcc* src, uint32_t njobs, hipStream_t s) {
+    if (njobs == 0) return hipSuccess;
+    hipLaunchKernelGGL(add_acc_kernel, dim3(njobs), dim3(256), 0, s, dst, src);
     return hipGetLastError();
 }
 

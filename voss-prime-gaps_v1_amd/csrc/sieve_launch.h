@@ -19,8 +19,10 @@ struct SieveArgs {
     uint32_t pat_step[kPresieveGroups];  // block size mod period
     const unsigned long long* lg_thresh; // [kHistBins]: largest p_before
                                          // with float merit >= 10, per g/2
-    const unsigned long long* bucket_off;  // first entry of each bucket
-    const uint32_t* bucket_cnt;            // entries of each bucket
+    const unsigned long long* bucket_off;  // first entry of each bucket, and
+                                           // one more: the end of the last
+    const uint32_t* bucket_cnt;            // entries of each bucket (values
+                                           // beyond the bucket are clamped)
     const uint32_t* bucket_ent;            // (offset << 3) | bit
     unsigned long long* emit_out;          // EMIT mode: primes, in order
     const unsigned long long* emit_off;    // EMIT mode: first slot per window
@@ -42,7 +44,8 @@ hipError_t launch_sieve_emit(const SieveArgs& a, int window_bytes, uint32_t grid
                              hipStream_t s);
 
 // Bucket passes over `ntask` tasks.  count: bucket_cnt[b] += hits.  fill:
-// writes entries using bucket_off and the per-bucket cursors.
+// writes entries using bucket_off (nbuckets + 1 offsets) and the per-bucket
+// cursors; a bucket that would exceed its room sets *overflow instead.
 hipError_t launch_bucket_count(const BucketTask* tasks, uint32_t ntask,
                                const PrimeRec* primes, uint32_t* bucket_cnt,
                                int log2w, uint32_t grid, hipStream_t s);
@@ -50,8 +53,11 @@ hipError_t launch_bucket_fill(const BucketTask* tasks, uint32_t ntask,
                               const PrimeRec* primes,
                               const unsigned long long* bucket_off,
                               uint32_t* bucket_cursor, uint32_t* entries,
-                              int log2w, uint32_t grid, hipStream_t s);
+                              uint32_t* overflow, int log2w, uint32_t grid,
+                              hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+// dst[j] += src[j] for j < njobs (sums; min first, max last, or error).
+hipError_t launch_add_acc(JobAcc* dst, const JobAcc* src, uint32_t njobs, hipStream_t s);
 int sieve_block_threads(int window_bytes);
 
 }  // namespace voss

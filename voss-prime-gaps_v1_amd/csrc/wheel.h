@@ -64,6 +64,8 @@ struct BucketTask {
     uint32_t nwin;
     uint32_t i0;
     uint32_t i1;
+    uint32_t walk;  // 1: thread per prime stepping over all its multiples
+    uint32_t pad;
 };
 constexpr int kBucketTileMax = 8192;  // windows per task (LDS counters)
 

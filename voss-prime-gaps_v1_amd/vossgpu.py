@@ -124,17 +124,19 @@ class ServeConfig(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("window_bytes", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32),
                 ("tier_a_limit", ctypes.c_uint32), ("bucket_limit", ctypes.c_uint32),
-                ("bucket_min_windows", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("bucket_min_windows", ctypes.c_uint32), ("bucket_room_pct", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
 
 
 EXPORTS = [
     "voss_create", "voss_destroy", "voss_last_error", "voss_device_name",
-    "voss_set_tuning", "voss_get_tuning", "voss_run_plain", "voss_run_plain_range",
+    "voss_set_tuning", "voss_get_tuning", "voss_autotune", "voss_run_plain", "voss_run_plain_range",
     "voss_sample_plain",
     "voss_run_interval", "voss_run_batch", "voss_run_batch_plain", "voss_mr_batch",
     "voss_mr_batch_device", "voss_verify_gaps", "voss_device_alloc", "voss_device_free",
     "voss_memcpy_h2d", "voss_memcpy_d2h", "voss_serve_start", "voss_serve_submit_interval",
     "voss_serve_submit_mr", "voss_serve_poll", "voss_serve_wait", "voss_serve_stop",
+    "voss_set_graph_mode", "voss_batch_prepare", "voss_batch_run", "voss_batch_free",
 ]
 
 _lib = None
@@ -157,12 +159,18 @@ def lib():
         L.voss_device_name.argtypes = [vp, ctypes.c_char_p, sz, P(i), P(ctypes.c_double)]
         L.voss_set_tuning.argtypes = [vp, P(Tuning)]
         L.voss_get_tuning.argtypes = [vp, P(Tuning)]
+        L.voss_autotune.argtypes = [vp, vp, vp, sz, sz, i, P(Tuning), P(ctypes.c_double)]
         L.voss_run_plain.argtypes = [vp, u64, u64, P(Result)]
         L.voss_sample_plain.argtypes = [vp, u64, u64, vp, vp, P(u64)]
         L.voss_run_plain_range.argtypes = [vp, u64, u64, u64, u64, P(Result)]
         L.voss_run_interval.argtypes = [vp, u64, u64, P(Result)]
         L.voss_run_batch.argtypes = [vp, vp, vp, sz, vp, P(Result)]
         L.voss_run_batch_plain.argtypes = [vp, vp, vp, sz, vp, P(Result)]
+        L.voss_set_graph_mode.argtypes = [vp, i]
+        L.voss_batch_prepare.argtypes = [vp, vp, vp, sz, P(vp)]
+        L.voss_batch_run.argtypes = [vp, vp, P(Result)]
+        L.voss_batch_free.argtypes = [vp]
+        L.voss_batch_free.restype = None
         L.voss_mr_batch.argtypes = [vp, vp, sz, vp]
         L.voss_mr_batch_device.argtypes = [vp, vp, sz, vp, P(ctypes.c_double)]
         L.voss_verify_gaps.argtypes = [vp, vp, vp, sz, vp]
@@ -177,7 +185,7 @@ def lib():
         L.voss_serve_wait.argtypes = [vp, u64, vp, vp, P(ctypes.c_double)]
         L.voss_serve_stop.argtypes = [vp]
         for name in EXPORTS:
-            if name not in ("voss_destroy", "voss_last_error"):
+            if name not in ("voss_destroy", "voss_last_error", "voss_batch_free"):
                 getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -259,12 +267,13 @@ class Engine:
         _check(lib().voss_get_tuning(self._h, ctypes.byref(t)))
         return {"window_bytes": t.window_bytes, "wg_per_cu": t.wg_per_cu,
                 "tier_a_limit": t.tier_a_limit, "bucket_limit": t.bucket_limit,
-                "bucket_min_windows": t.bucket_min_windows}
+                "bucket_min_windows": t.bucket_min_windows, "bucket_room_pct": t.bucket_room_pct}
 
     def set_tuning(self, window_bytes=0, wg_per_cu=0, tier_a_limit=0, bucket_limit=0,
-                   bucket_min_windows=0):
+                   bucket_min_windows=0, bucket_room_pct=0):
         """Zero fields keep the current value."""
-        t = Tuning(window_bytes, wg_per_cu, tier_a_limit, bucket_limit, bucket_min_windows)
+        t = Tuning(window_bytes, wg_per_cu, tier_a_limit, bucket_limit, bucket_min_windows,
+                   bucket_room_pct)
         _check(lib().voss_set_tuning(self._h, ctypes.byref(t)))
 
     def _result(self, large_cap):
@@ -277,6 +286,17 @@ class Engine:
         r = self._res
         r.large_capacity = large_cap
         return r
+
+    def autotune(self, lo, hi, max_intervals=512, repeats=1):
+        """Coordinate-descent autotuner on a sample of the intervals; the best
+        tuning is installed and returned with its time."""
+        lo = np.ascontiguousarray(lo, dtype=np.uint64)
+        hi = np.ascontiguousarray(hi, dtype=np.uint64)
+        t = Tuning()
+        sec = ctypes.c_double()
+        _check(lib().voss_autotune(self._h, lo.ctypes.data, hi.ctypes.data, lo.shape[0],
+                                   max_intervals, repeats, ctypes.byref(t), ctypes.byref(sec)))
+        return self.tuning(), sec.value
 
     def run_plain(self, N, seg_num, large_cap=1 << 20):
         r = self._result(large_cap)
@@ -303,10 +323,8 @@ class Engine:
         _check(lib().voss_run_interval(self._h, lo, hi, ctypes.byref(r)))
         return result_to_dict(r)
 
-    def _batch(self, fn, lo, hi, large_cap, raw):
-        lo = np.ascontiguousarray(lo, dtype=np.uint64)
-        hi = np.ascontiguousarray(hi, dtype=np.uint64)
-        n = lo.shape[0]
+    @staticmethod
+    def _interval_results(n, large_cap):
         res = (IntervalResult * n)()
         lbuf = (LargeGap * max(large_cap * n, 1))()
         base = ctypes.addressof(lbuf)
@@ -314,15 +332,35 @@ class Engine:
             if large_cap:
                 res[k].large = ctypes.cast(base + 16 * large_cap * k, ctypes.POINTER(LargeGap))
                 res[k].large_capacity = large_cap
+        return res, lbuf
+
+    @staticmethod
+    def _stats(stats):
+        return {"seconds": stats.seconds, "device_ms": stats.device_ms,
+                "sieve_ms": stats.sieve_ms, "windows": int(stats.windows),
+                "bucket_ms": stats.bucket_ms, "bucket_entries": int(stats.bucket_entries)}
+
+    def _batch(self, fn, lo, hi, large_cap, raw):
+        lo = np.ascontiguousarray(lo, dtype=np.uint64)
+        hi = np.ascontiguousarray(hi, dtype=np.uint64)
+        n = lo.shape[0]
+        res, lbuf = self._interval_results(n, large_cap)
         stats = Result()
         _check(fn(self._h, lo.ctypes.data, hi.ctypes.data, n, ctypes.cast(res, ctypes.c_void_p),
                   ctypes.byref(stats)))
-        st = {"seconds": stats.seconds, "device_ms": stats.device_ms,
-              "sieve_ms": stats.sieve_ms, "windows": int(stats.windows),
-              "bucket_ms": stats.bucket_ms, "bucket_entries": int(stats.bucket_entries)}
+        st = self._stats(stats)
         if raw:
             return res, lbuf, st
         return [interval_to_dict(res[k]) for k in range(n)], st
+
+    def set_graph_mode(self, enable):
+        """True: voss_run_batch runs as one captured hipGraph (the default)."""
+        _check(lib().voss_set_graph_mode(self._h, 1 if enable else 0))
+
+    def prepare_batch(self, lo, hi):
+        """A prepared batch: planned once, captured on its first run and
+        replayed by every later run."""
+        return PreparedBatch(self, lo, hi)
 
     def run_batch(self, lo, hi, large_cap=64, raw=False):
         """All intervals in one persistent launch.  Returns (results, stats);
@@ -374,6 +412,39 @@ class Engine:
         _check(lib().voss_verify_gaps(self._h, pa.ctypes.data, g.ctypes.data, pa.shape[0],
                                       ok.ctypes.data))
         return ok
+
+
+class PreparedBatch:
+    """voss_batch_prepare / voss_batch_run / voss_batch_free."""
+
+    def __init__(self, engine, lo, hi):
+        self.engine = engine
+        self.lo = np.ascontiguousarray(lo, dtype=np.uint64)
+        self.hi = np.ascontiguousarray(hi, dtype=np.uint64)
+        self._b = ctypes.c_void_p()
+        _check(lib().voss_batch_prepare(engine._h, self.lo.ctypes.data, self.hi.ctypes.data,
+                                        self.lo.shape[0], ctypes.byref(self._b)))
+
+    def run(self, large_cap=64, raw=False):
+        n = self.lo.shape[0]
+        res, lbuf = Engine._interval_results(n, large_cap)
+        stats = Result()
+        _check(lib().voss_batch_run(self._b, ctypes.cast(res, ctypes.c_void_p), ctypes.byref(stats)))
+        st = Engine._stats(stats)
+        if raw:
+            return res, lbuf, st
+        return [interval_to_dict(res[k]) for k in range(n)], st
+
+    def close(self):
+        if self._b:
+            lib().voss_batch_free(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Server:
