@@ -269,6 +269,10 @@ def main():
                     help="only the 1e11 steps and MR (for profiling runs)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the per-interval comparison form of the batch leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the batch leg with the default tuning")
+    ap.add_argument("--legs", default="interval,batch,serve",
+                    help="comma-separated legs after the headline: interval, batch, serve")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -299,19 +303,27 @@ def main():
 
     import workloads as W
     legs = {}
-    if not args.headline_only:
+    want = set() if args.headline_only else set(args.legs.split(","))
+    ends = np.array([10**12 + 39], dtype=np.uint64)
+    if "interval" in want:
         legs["interval_1e13"] = interval_leg(eng)
-        legs["batch_4096"], ends = batch_leg(eng, W, with_plain=not args.no_plain)
+    if "batch" in want:
+        legs["batch_4096"], ends = batch_leg(eng, W, with_plain=not args.no_plain,
+                                             autotune=not args.no_autotune)
+    if "serve" in want:
         legs["serve"] = serve_leg(eng, W)
-    else:
-        ends = np.array([10**12 + 39], dtype=np.uint64)
 
     # Miller-Rabin tests/s on device-resident candidates (config 4's set)
     cand = W.mr_candidates(args.mr_count, ends)
     verdict, mr_ms = eng.mr_device(cand, repeats=3)
     mr_rate = len(cand) / (statistics.median(mr_ms) / 1e3)
+    eng.mr(cand[:1000])
+    t = time.perf_counter()
+    v_host = eng.mr(cand)  # host buffers: both PCIe copies inside the timing
+    t_host = time.perf_counter() - t
+    assert np.array_equal(v_host, verdict)
     mr = {"tests_per_s": mr_rate, "count": int(len(cand)), "primes": int(verdict.sum()),
-          "kernel_ms": mr_ms}
+          "kernel_ms": mr_ms, "host_buffers_tests_per_s": len(cand) / t_host}
     if not args.no_cpu:
         from oracle import oracle as O
         threads = min(16, os.cpu_count() or 1)

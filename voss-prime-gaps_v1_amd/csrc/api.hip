@@ -173,7 +173,7 @@ struct voss_ctx {
     std::string name;
     int num_cus = 0;
     double total_gb = 0;
-    voss_tuning tune{131072, 1, 1024, 1u << 21, 4, 100, {0, 0}};
+    voss_tuning tune{131072, 1, 1024, 1u << 21, 2, 100, {0, 0}};
 
     std::vector<uint32_t> primes;
     std::vector<double> inv_prefix;  // inv_prefix[i] = sum of 1/p over primes[0..i)
@@ -378,25 +378,43 @@ void add_acc(JobHost& J, const JobAcc& A) {
 // Canonical order of the large-gap list: merit descending, then position
 // ascending (the reference sorts by merit alone with an unstable sort,
 // src/voss_v7_live.cu:813, so ties have no defined order there).  Merits
-// are positive floats, whose bit patterns order like the values, so one
-// 128-bit key (inverted merit bits : position) sorts both at once.
+// are positive floats, whose bit patterns order like the values: an LSD
+// radix sort of the 64-bit keys (inverted merit bits : list index), byte
+// passes whose digit never varies skipped, then runs of equal merit sorted
+// by position.
 void sort_large(std::vector<voss_large_gap>& v) {
-    struct K {
-        unsigned __int128 key;
-        int32_t gap;
-        float merit;
-    };
-    std::vector<K> k(v.size());
-    for (size_t i = 0; i < v.size(); i++) {
+    const size_t n = v.size();
+    if (n < 2) return;
+    std::vector<uint64_t> k(n), t(n);
+    uint64_t all_or = 0, all_and = ~0ull;
+    for (size_t i = 0; i < n; i++) {
         uint32_t mb;
         std::memcpy(&mb, &v[i].merit, 4);
-        k[i].key = ((unsigned __int128)(0xFFFFFFFFu - mb) << 64) | v[i].position;
-        k[i].gap = v[i].gap;
-        k[i].merit = v[i].merit;
+        k[i] = ((uint64_t)(0xFFFFFFFFu - mb) << 32) | (uint64_t)i;
+        all_or |= k[i];
+        all_and &= k[i];
     }
-    std::sort(k.begin(), k.end(), [](const K& a, const K& b) { return a.key < b.key; });
-    for (size_t i = 0; i < v.size(); i++)
-        v[i] = {(uint64_t)k[i].key, k[i].gap, k[i].merit};
+    for (int sh = 0; sh < 64; sh += 8) {
+        if ((((all_or ^ all_and) >> sh) & 0xFF) == 0) continue;  // digit constant
+        size_t cnt[257] = {};
+        for (size_t i = 0; i < n; i++) cnt[((k[i] >> sh) & 0xFF) + 1]++;
+        for (int d = 0; d < 256; d++) cnt[d + 1] += cnt[d];
+        for (size_t i = 0; i < n; i++) t[cnt[(k[i] >> sh) & 0xFF]++] = k[i];
+        k.swap(t);
+    }
+    std::vector<voss_large_gap> out(n);
+    for (size_t i = 0; i < n; i++) out[i] = v[(uint32_t)k[i]];
+    for (size_t i = 0; i < n;) {
+        size_t j = i + 1;
+        while (j < n && (k[j] >> 32) == (k[i] >> 32)) j++;
+        if (j - i > 1)
+            std::sort(out.begin() + i, out.begin() + j,
+                      [](const voss_large_gap& x, const voss_large_gap& y) {
+                          return x.position < y.position;
+                      });
+        i = j;
+    }
+    v.swap(out);
 }
 
 // Histograms and large gaps into the reference's array layout.

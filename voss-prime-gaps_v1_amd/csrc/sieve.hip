@@ -304,23 +304,51 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
         const uint32_t per = L4 / BLOCK;
         const uint32_t i0 = (uint32_t)tid * per;
         const uint32_t iend = i0 + per;
-        uint32_t cnt = 0, c1 = 0, m30[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        // pre-pass: clip the run to [n_lo, n_hi] and count by residue
-        for (uint32_t i = i0; i < iend; i++) {
-            const uint32_t w0 = bm[i];
-            uint32_t w = w0;
-            const int lo = (int)(32u * i);
-            if (lo < bitlo) w &= (bitlo - lo >= 32) ? 0u : (0xFFFFFFFFu << (bitlo - lo));
-            if (lo + 31 > bithi)
-                w &= (bithi < lo) ? 0u
-                     : (bithi - lo >= 31 ? 0xFFFFFFFFu : ((2u << (bithi - lo)) - 1u));
-            if (w != w0) bm[i] = w;
-            cnt += __popc(w);
-            // bytes 0 and 2 of a word have even k (w0 % 4 == 0):
-            // p%4 == 1 for residues {1,13,17,29} at even k, {7,11,19,23} at odd k
-            c1 += __popc(w & 0x66996699u);
+        // pre-pass: clip the run to [n_lo, n_hi] (only runs at the window's
+        // ends need it) and add its words into bit-sliced counters: bit j
+        // of v[k] is bit k of the number of words with bit j set (a run has
+        // at most 32 words, so six planes hold every count)
+        uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+        auto add_word = [&](uint32_t w) {
+            uint32_t c = w, t;
+            t = v0 & c; v0 ^= c; c = t;
+            t = v1 & c; v1 ^= c; c = t;
+            t = v2 & c; v2 ^= c; c = t;
+            t = v3 & c; v3 ^= c; c = t;
+            t = v4 & c; v4 ^= c; c = t;
+            v5 ^= c;
+        };
+        if ((int)(32u * i0) < bitlo || (int)(32u * iend) - 1 > bithi) {
+            for (uint32_t i = i0; i < iend; i++) {
+                const uint32_t w0 = bm[i];
+                uint32_t w = w0;
+                const int lo = (int)(32u * i);
+                if (lo < bitlo) w &= (bitlo - lo >= 32) ? 0u : (0xFFFFFFFFu << (bitlo - lo));
+                if (lo + 31 > bithi)
+                    w &= (bithi < lo) ? 0u
+                         : (bithi - lo >= 31 ? 0xFFFFFFFFu : ((2u << (bithi - lo)) - 1u));
+                if (w != w0) bm[i] = w;
+                add_word(w);
+            }
+        } else {
+            for (uint32_t i = i0; i < iend; i++) add_word(bm[i]);
+        }
+        // counts from the planes: all primes, p % 4 == 1 (bytes 0 and 2 of a
+        // word have even k: residues {1,13,17,29} there, {7,11,19,23} at odd
+        // k), and per wheel residue
+        uint32_t cnt = 0, c1 = 0, m30[8];
 #pragma unroll
-            for (int o = 0; o < 8; o++) m30[o] += __popc(w & (0x01010101u << o));
+        for (int o = 0; o < 8; o++) m30[o] = 0;
+        {
+            const uint32_t vp[6] = {v0, v1, v2, v3, v4, v5};
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                cnt += (uint32_t)__popc(vp[k]) << k;
+                c1 += (uint32_t)__popc(vp[k] & 0x66996699u) << k;
+#pragma unroll
+                for (int o = 0; o < 8; o++)
+                    m30[o] += (uint32_t)__popc(vp[k] & (0x01010101u << o)) << k;
+            }
         }
         // EMIT: exclusive prefix of the per-thread counts over the block
         // gives each thread its first output slot (ordered compaction)
@@ -340,13 +368,22 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             __syncthreads();
         }
         // walk: primes of the run in increasing order, word by word.  The
-        // first prime of the run pairs with itself (gap 0), which lands in
-        // the unused bin 0 instead of taking a branch.
+        // first prime of the run pairs with itself (gap 0, into the unused
+        // bin 0), so the loop body has no first-prime case.
         uint32_t first = 0, prev = 0;
         {
+            uint32_t i = i0;
+            while (i < iend && bm[i] == 0u) i++;
             uint32_t prevoff = 0, rowb = 0;  // rowb = 4 * wheel bit of p_before
+            if (i < iend) {
+                const uint32_t b = __builtin_ctz(bm[i]);
+                prevoff = 120u * i + 30u * (b >> 3) +
+                          __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, (b & 7u) | 0x0C0C0C00u);
+                first = prevoff + 1u;
+                prev = cnt ? 1u : 0u;
+            }
             char* const mqb = (char*)mq;
-            for (uint32_t i = i0; i < iend; i++) {
+            for (; i < iend; i++) {
                 uint32_t w = bm[i];
                 if (!w) continue;
                 const uint32_t base = 120u * i;
@@ -360,18 +397,14 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     const uint32_t off = base + 30u * (b >> 3) + w30;
                     if constexpr (EMIT) {
                         a.emit_out[emit_at++] = base_num + off;
-                        first = first ? first : off + 1u;
-                        prev = 1u;
                         prevoff = off;
                         continue;
                     }
-                    const uint32_t g = prev ? off - prevoff : 0u;
-                    first = first ? first : off + 1u;
+                    const uint32_t g = off - prevoff;
                     // byte address 4*(8*(g/2) + row) = 16*g + 4*row (g even):
                     // equal gaps of different rows fall in different banks
                     if (g < 100u) atomicAdd((uint32_t*)(mqb + rowb + 16u * g), 1u);
                     else pair_wide(pc, prevoff, g);
-                    prev = 1u;
                     prevoff = off;
                     rowb = (b & 7u) << 2;
                 } while (w);
