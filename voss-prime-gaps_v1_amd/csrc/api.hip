@@ -458,6 +458,8 @@ void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job
     const uint64_t ka = (a / 30 - grid_origin) / W, kb = (b / 30 - grid_origin) / W;
     size_t np = 0;
     const size_t nall = c->primes.size();
+    const uint32_t ibig_w =
+        (uint32_t)(std::lower_bound(c->primes.begin(), c->primes.end(), W) - c->primes.begin());
     for (uint64_t k = ka; k <= kb; k++) {
         const uint64_t w0 = grid_origin + k * W;
         const uint64_t lo = std::max<uint64_t>(a, 30 * w0);
@@ -477,7 +479,9 @@ void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job
         d.job = job;
         d.lseg = lseg;
         d.bucket = kNoBucket;
-        d.pad = 0;
+        d.ibig = d.len == W ? ibig_w
+                            : (uint32_t)(std::lower_bound(c->primes.begin(), c->primes.end(),
+                                                          d.len) - c->primes.begin());
         P.desc.push_back(d);
     }
 }
@@ -844,7 +848,7 @@ struct Executor {
                 unsigned long long o = 0;
                 for (uint32_t b = 0; b < R.nbuckets; b++) {
                     B->h_boff.p[R.boff_off + b] = o;
-                    R.room[b] = B->h_bcnt.p[b];
+                    R.room[b] = (B->h_bcnt.p[b] + 3u) & ~3u;  // 16-byte aligned buckets
                     o += R.room[b];
                 }
                 B->h_boff.p[R.boff_off + R.nbuckets] = o;
@@ -1242,7 +1246,8 @@ int voss_autotune(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
     const std::vector<std::pair<uint32_t, uint32_t>> shapes = {
         {131072, 1}, {65536, 2}, {65536, 1}, {32768, 4}};
     const std::vector<uint32_t> tiers = {512, 1024, 2048};
-    const std::vector<uint32_t> blimits = {1u << 21, 1u << 23, 1u << 25, 0xFFFFFFFFu};
+    const std::vector<uint32_t> blimits = {1u << 17, 1u << 19, 1u << 21, 1u << 23, 1u << 25,
+                                           0xFFFFFFFFu};
     const std::vector<uint32_t> bmins = {2, 4, 8};
     for (int pass = 0; pass < 2; pass++) {
         bool moved = false;

@@ -94,6 +94,51 @@ __device__ __forceinline__ uint32_t first_pos(uint32_t p, uint32_t r,
     return pos;
 }
 
+// wheel bit of residue r (r coprime to 30), four bits per residue
+constexpr unsigned long long res_nibbles(int half) {
+    unsigned long long v = 0;
+    for (int o = 0; o < 8; o++) {
+        const int r = (int)((kW30Bytes >> (8 * o)) & 0xFF);
+        if ((r >> 4) == half) v |= (unsigned long long)o << (4 * (r & 15));
+    }
+    return v;
+}
+constexpr unsigned long long kResLo = res_nibbles(0), kResHi = res_nibbles(1);
+constexpr uint32_t kCoprime30 = (1u << 1) | (1u << 7) | (1u << 11) | (1u << 13) | (1u << 17) |
+                                (1u << 19) | (1u << 23) | (1u << 29);
+__device__ __forceinline__ uint32_t wheel_bit_of_residue(uint32_t r) {
+    return (uint32_t)(((r < 16 ? kResLo : kResHi) >> (4u * (r & 15u))) & 7u);
+}
+
+
+// Marks m*p for every m >= 2 coprime to 30 with m*p in the window
+// [30*w0, 30*(w0+len)): one Barrett reduction for the first multiple, then
+// m -> m+1 steps (p/30 bytes and p%30 in the residue), keeping those
+// coprime to 30.  For p >= 30*len/7 this visits at most ~5 multiples, fewer
+// than the 8 residue-class starts of the class-by-class marking.
+__device__ __forceinline__ void walk_prime(uint32_t* bm, uint32_t p, unsigned long long magic,
+                                           uint64_t w0, uint32_t len) {
+    const uint64_t X = 30ull * w0;
+    uint32_t x;
+    if (X < 2ull * p) {
+        x = (uint32_t)(2ull * p - X);
+    } else {
+        const uint32_t r = mod_u64(X, p, magic);
+        x = r ? p - r : 0u;
+    }
+    uint32_t b = x / 30u, rr = x - 30u * b;
+    const uint32_t pd = p / 30u, pm = p - 30u * pd;
+    while (b < len) {
+        if ((kCoprime30 >> rr) & 1u) clear_bit(bm, b, wheel_bit_of_residue(rr));
+        b += pd;
+        rr += pm;
+        if (rr >= 30u) {
+            rr -= 30u;
+            b++;
+        }
+    }
+}
+
 // Thresholds of the merit rule for the even gaps 102..500 (bins 51..250)
 // live in LDS: lgt[h - 51].
 constexpr int kLgtFirst = 51, kLgtCount = 200;
@@ -231,18 +276,32 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             const uint32_t np = D.nprimes;
             const uint32_t a_end = np < a.tier_a_end ? np : a.tier_a_end;
             const uint32_t nA = a_end > a.tier_a_begin ? a_end - a.tier_a_begin : 0;
-            const uint32_t nB = np > a.tier_a_end ? (np - a.tier_a_end + 63) / 64 : 0;
-            const uint32_t ntask = nA + nB;
+            // tier B1: primes below len, 64 per task; tier B2: primes >= len
+            // (a few hits each), 256 per task with four record loads in
+            // flight per lane, so the loads' latency is paid once per task
+            const uint32_t b2 = max(a.tier_a_end, min(np, D.ibig));
+            const uint32_t nB1 = b2 > a.tier_a_end ? (b2 - a.tier_a_end + 63) / 64 : 0;
+            const uint32_t nB2 = np > b2 ? (np - b2 + 255) / 256 : 0;
+            const uint32_t ntask = nA + nB1 + nB2;
             const uint32_t len = D.len;
             if (D.bucket != kNoBucket) {
-                // large primes: their hits in this window, precomputed
+                // large primes: their hits in this window, precomputed.
+                // Buckets start 16-byte aligned and their room is a multiple
+                // of 4 entries, so every lane reads 4 entries with one load.
                 const unsigned long long e0 = a.bucket_off[D.bucket];
                 const unsigned long long cap = a.bucket_off[D.bucket + 1] - e0;
                 const uint32_t ne = (uint32_t)min((unsigned long long)a.bucket_cnt[D.bucket], cap);
-                for (uint32_t k = tid; k < ne; k += BLOCK) {
-                    const uint32_t e = a.bucket_ent[e0 + k];
-                    const uint32_t b = e >> 3;
-                    atomicAnd(&bm[b >> 2], ~(1u << (((b & 3u) << 3) | (e & 7u))));
+                const uint4* ent4 = reinterpret_cast<const uint4*>(a.bucket_ent + e0);
+                for (uint32_t k = tid; 4u * k < ne; k += BLOCK) {
+                    const uint4 e4 = ent4[k];
+                    const uint32_t ev[4] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (4u * k + q < ne) {
+                            const uint32_t b = ev[q] >> 3;
+                            atomicAnd(&bm[b >> 2], ~(1u << (((b & 3u) << 3) | (ev[q] & 7u))));
+                        }
+                    }
                 }
             }
             while (true) {
@@ -262,25 +321,39 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     const uint32_t stride = 8u * p;  // = 0 mod 4: one mask
                     const uint32_t m = clear_mask(b, bit);
                     for (; b < len; b += stride) atomicAnd(&bm[b >> 2], m);
-                } else {
+                } else if (t < nA + nB1) {
                     const uint32_t pi = a.tier_a_end + (t - nA) * 64 + lane;
-                    if (pi < np) {
+                    if (pi < b2) {
                         const PrimeRec pr = a.primes[pi];
                         const uint32_t p = pr.p;
                         const uint32_t r = mod_u64(D.w0, p, pr.magic);
-                        const uint32_t perm = pr.perm;
-                        if (p >= len) {
+#pragma unroll 1
+                        for (int j = 0; j < 8; j++)
+                            cross_class(bm, first_pos(p, r, j, D.w0), p,
+                                        (pr.perm >> (3 * j)) & 7u, len);
+                    }
+                } else {
+                    const uint32_t pi0 = b2 + (t - nA - nB1) * 256 + lane;
+                    PrimeRec pr[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t pi = pi0 + 64u * q;
+                        pr[q] = a.primes[pi < np ? pi : np - 1u];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (pi0 + 64u * q >= np) continue;
+                        const uint32_t p = pr[q].p;
+                        if (7ull * p >= 30ull * len) {
+                            walk_prime(bm, p, pr[q].magic, D.w0, len);
+                        } else {
                             // at most one multiple per class in the window
+                            const uint32_t r = mod_u64(D.w0, p, pr[q].magic);
 #pragma unroll
                             for (int j = 0; j < 8; j++) {
                                 const uint32_t b = first_pos(p, r, j, D.w0);
-                                if (b < len) clear_bit(bm, b, (perm >> (3 * j)) & 7u);
+                                if (b < len) clear_bit(bm, b, (pr[q].perm >> (3 * j)) & 7u);
                             }
-                        } else {
-#pragma unroll 1
-                            for (int j = 0; j < 8; j++)
-                                cross_class(bm, first_pos(p, r, j, D.w0), p,
-                                            (perm >> (3 * j)) & 7u, len);
                         }
                     }
                 }
@@ -538,28 +611,12 @@ __global__ void init_acc_kernel(JobAcc* acc, uint32_t njobs) {
 // writes nothing and sets *overflow (the caller then re-runs with exact
 // counts), so no write ever leaves the bucket.
 
-// wheel bit of residue r (r coprime to 30), four bits per residue
-constexpr unsigned long long res_nibbles(int half) {
-    unsigned long long v = 0;
-    for (int o = 0; o < 8; o++) {
-        const int r = (int)((kW30Bytes >> (8 * o)) & 0xFF);
-        if ((r >> 4) == half) v |= (unsigned long long)o << (4 * (r & 15));
-    }
-    return v;
-}
-constexpr unsigned long long kResLo = res_nibbles(0), kResHi = res_nibbles(1);
-constexpr uint32_t kCoprime30 = (1u << 1) | (1u << 7) | (1u << 11) | (1u << 13) | (1u << 17) |
-                                (1u << 19) | (1u << 23) | (1u << 29);
-__device__ __forceinline__ uint32_t wheel_bit_of_residue(uint32_t r) {
-    return (uint32_t)(((r < 16 ? kResLo : kResHi) >> (4u * (r & 15u))) & 7u);
-}
-
-// Calls f(byte offset in the tile, wheel bit) for every hit of work item k.
+// Calls f(byte offset in the tile, wheel bit) for every hit of work item k
+// (record pr already loaded: prime k in walk mode, prime k/8 otherwise).
 template <typename F>
-__device__ __forceinline__ void bucket_hits(const BucketTask& T, const PrimeRec* __restrict__ primes,
-                                            uint32_t k, uint32_t span, F&& f) {
+__device__ __forceinline__ void bucket_hits(const BucketTask& T, const PrimeRec& pr, uint32_t k,
+                                            uint32_t span, F&& f) {
     if (T.walk) {
-        const PrimeRec pr = primes[T.i0 + k];
         const uint32_t p = pr.p;
         const uint64_t X = 30ull * T.w0;  // first number of the tile
         uint32_t x;                       // first multiple m*p >= X with m >= 2, minus X
@@ -581,12 +638,76 @@ __device__ __forceinline__ void bucket_hits(const BucketTask& T, const PrimeRec*
             }
         }
     } else {
-        const PrimeRec pr = primes[T.i0 + (k >> 3)];
         const int j = (int)(k & 7u);
-        const uint32_t r = mod_u64(T.w0, pr.p, pr.magic);
+        const uint32_t p = pr.p;
+        const uint32_t r = mod_u64(T.w0, p, pr.magic);
         const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
-        for (uint32_t b = first_pos(pr.p, r, j, T.w0); b < span; b += pr.p) f(b, bit);
+        for (uint32_t b = first_pos(p, r, j, T.w0); b < span; b += p) f(b, bit);
     }
+}
+
+// FILL in class mode, four hits per step: the four slot reservations (LDS
+// atomics with return) and the four room reads are issued before any result
+// is used, so their latencies overlap instead of adding up per hit.
+__device__ __forceinline__ void bucket_fill_class(const BucketTask& T, const PrimeRec& pr,
+                                                  uint32_t k, uint32_t span, uint32_t* cnt,
+                                                  const uint32_t* lim, uint32_t* entries,
+                                                  int log2w, uint32_t wmask) {
+    const int j = (int)(k & 7u);
+    const uint32_t p = pr.p;
+    const uint32_t r = mod_u64(T.w0, p, pr.magic);
+    const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
+    uint32_t b = first_pos(p, r, j, T.w0);
+    while ((uint64_t)b + 3ull * p < span) {
+        uint32_t bb[4], sl[4], lm[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            bb[q] = b + (uint32_t)q * p;
+            sl[q] = atomicAdd(&cnt[bb[q] >> log2w], 1u);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) lm[q] = lim[bb[q] >> log2w];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (sl[q] < lm[q]) entries[sl[q]] = ((bb[q] & wmask) << 3) | bit;
+        b += 4u * p;
+    }
+    for (; b < span; b += p) {
+        const uint32_t w = b >> log2w;
+        const uint32_t slot = atomicAdd(&cnt[w], 1u);
+        if (slot < lim[w]) entries[slot] = ((b & wmask) << 3) | bit;
+    }
+}
+
+// Every work item of a task, four per thread at a time: the four prime
+// records are loaded before any is used, so one load latency covers four
+// items (the items are short: the loads would otherwise dominate).
+template <typename G>
+__device__ __forceinline__ void bucket_sweep_items(const BucketTask& T,
+                                                   const PrimeRec* __restrict__ primes, G&& g) {
+    const uint32_t nthr = (T.i1 - T.i0) * (T.walk ? 1u : 8u);
+    const uint32_t last = T.i1 - 1u;
+    for (uint32_t k0 = threadIdx.x; k0 < nthr; k0 += 4u * blockDim.x) {
+        PrimeRec pr[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t k = k0 + q * blockDim.x;
+            const uint32_t i = T.i0 + (T.walk ? k : k >> 3);
+            pr[q] = primes[i < last ? i : last];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t k = k0 + q * blockDim.x;
+            if (k < nthr) g(pr[q], k);
+        }
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ void bucket_sweep(const BucketTask& T, const PrimeRec* __restrict__ primes,
+                                             uint32_t span, F&& f) {
+    bucket_sweep_items(T, primes,
+                       [&](const PrimeRec& pr, uint32_t k) { bucket_hits(T, pr, k, span, f); });
 }
 
 template <bool FILL>
@@ -607,10 +728,8 @@ __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restri
         const uint32_t span = T.nwin << log2w;  // bytes; at most 2^30
         for (uint32_t i = tid; i < T.nwin; i += blockDim.x) cnt[i] = 0;
         __syncthreads();
-        const uint32_t nthr = (T.i1 - T.i0) * (T.walk ? 1u : 8u);
-        for (uint32_t k = tid; k < nthr; k += blockDim.x)
-            bucket_hits(T, primes, k, span,
-                        [&](uint32_t b, uint32_t) { atomicAdd(&cnt[b >> log2w], 1u); });
+        bucket_sweep(T, primes, span,
+                     [&](uint32_t b, uint32_t) { atomicAdd(&cnt[b >> log2w], 1u); });
         __syncthreads();
         if (!FILL) {
             for (uint32_t i = tid; i < T.nwin; i += blockDim.x)
@@ -627,12 +746,17 @@ __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restri
             lim[i] = (uint32_t)o1;
         }
         __syncthreads();
-        for (uint32_t k = tid; k < nthr; k += blockDim.x)
-            bucket_hits(T, primes, k, span, [&](uint32_t b, uint32_t bit) {
+        if (T.walk) {
+            bucket_sweep(T, primes, span, [&](uint32_t b, uint32_t bit) {
                 const uint32_t w = b >> log2w;
                 const uint32_t slot = atomicAdd(&cnt[w], 1u);
                 if (slot < lim[w]) entries[slot] = ((b & wmask) << 3) | bit;
             });
+        } else {
+            bucket_sweep_items(T, primes, [&](const PrimeRec& pr, uint32_t k) {
+                bucket_fill_class(T, pr, k, span, cnt, lim, entries, log2w, wmask);
+            });
+        }
         __syncthreads();
     }
 }

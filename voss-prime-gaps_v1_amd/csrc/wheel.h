@@ -51,7 +51,8 @@ struct SegDesc {
     uint32_t job;      // output slot (interval index in batch mode)
     uint32_t lseg;     // logical segment id (plain mode)
     uint32_t bucket;   // bucket of this window's large-prime hits, or kNoBucket
-    uint32_t pad;
+    uint32_t ibig;     // index of the first base prime >= len (at most one
+                       // hit per residue class in the window from there on)
 };
 constexpr uint32_t kNoBucket = 0xFFFFFFFFu;
 
