@@ -61,23 +61,36 @@ __device__ __forceinline__ uint32_t clear_mask(uint32_t b, uint32_t bit) {
     return ~(1u << (((b & 3u) << 3) | bit));
 }
 
-// Crosses bytes b, b+p, b+2p, ... < len of one residue class.  p is odd, so
-// (b & 3) repeats with period 4 along the class: the four word masks are
-// computed once and the loop body is one address and one ds_and per mark.
+// ds_and_b32 at LDS byte address a (a multiple of 4) of the bitmap.
+__device__ __forceinline__ void and_at(uint32_t* bm, uint32_t a, uint32_t m) {
+    atomicAnd(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(bm) + a), m);
+}
+
+// Crosses bytes b, b+p, b+2p, ... < len of one residue class (len is a
+// multiple of 4, so byte b < len iff its word address b & ~3 < len).  Four
+// consecutive marks are one step: each keeps its own word address, which
+// advances by 4p (a multiple of 4, so it stays word-aligned and exact), and
+// its own mask.  p is odd, so from one mark to the next the marked byte
+// moves by p mod 4 within the word: the next mask is the previous one
+// rotated by 8 * (p mod 4) bits.
 __device__ __forceinline__ void cross_class(uint32_t* bm, uint32_t b, uint32_t p,
                                             uint32_t bit, uint32_t len) {
-    const uint32_t m0 = clear_mask(b, bit), m1 = clear_mask(b + p, bit);
-    const uint32_t m2 = clear_mask(b + 2 * p, bit), m3 = clear_mask(b + 3 * p, bit);
-    const uint32_t p4 = 4 * p;
-    for (; b + 3 * p < len; b += p4) {
-        atomicAnd(&bm[b >> 2], m0);
-        atomicAnd(&bm[(b + p) >> 2], m1);
-        atomicAnd(&bm[(b + 2 * p) >> 2], m2);
-        atomicAnd(&bm[(b + 3 * p) >> 2], m3);
+    const uint32_t rot = 32u - 8u * (p & 3u);  // rotate right by this = left by 8*(p%4)
+    const uint32_t m0 = clear_mask(b, bit);
+    const uint32_t m1 = __builtin_amdgcn_alignbit(m0, m0, rot);
+    const uint32_t m2 = __builtin_amdgcn_alignbit(m1, m1, rot);
+    const uint32_t m3 = __builtin_amdgcn_alignbit(m2, m2, rot);
+    uint32_t a0 = b & ~3u, a1 = (b + p) & ~3u, a2 = (b + 2u * p) & ~3u, a3 = (b + 3u * p) & ~3u;
+    const uint32_t p4 = 4u * p;
+    for (; a3 < len; a0 += p4, a1 += p4, a2 += p4, a3 += p4) {
+        and_at(bm, a0, m0);
+        and_at(bm, a1, m1);
+        and_at(bm, a2, m2);
+        and_at(bm, a3, m3);
     }
-    if (b < len) atomicAnd(&bm[b >> 2], m0);
-    if (b + p < len) atomicAnd(&bm[(b + p) >> 2], m1);
-    if (b + 2 * p < len) atomicAnd(&bm[(b + 2 * p) >> 2], m2);
+    if (a0 < len) and_at(bm, a0, m0);
+    if (a1 < len) and_at(bm, a1, m1);
+    if (a2 < len) and_at(bm, a2, m2);
 }
 
 // Position of the first multiple of p in residue class j inside the window
@@ -317,10 +330,19 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     const int j = lane >> 3;
                     const uint32_t s = lane & 7;
                     const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
-                    uint32_t b = first_pos(p, r, j, D.w0) + s * p;
-                    const uint32_t stride = 8u * p;  // = 0 mod 4: one mask
+                    const uint32_t b = first_pos(p, r, j, D.w0) + s * p;
+                    // stride 8p = 0 mod 4: one mask, and the word address
+                    // b & ~3 advances exactly by the stride
+                    const uint32_t stride = 8u * p;
                     const uint32_t m = clear_mask(b, bit);
-                    for (; b < len; b += stride) atomicAnd(&bm[b >> 2], m);
+                    uint32_t a = b & ~3u;
+                    for (; a + 3u * stride < len; a += 4u * stride) {
+                        and_at(bm, a, m);
+                        and_at(bm, a + stride, m);
+                        and_at(bm, a + 2u * stride, m);
+                        and_at(bm, a + 3u * stride, m);
+                    }
+                    for (; a < len; a += stride) and_at(bm, a, m);
                 } else if (t < nA + nB1) {
                     const uint32_t pi = a.tier_a_end + (t - nA) * 64 + lane;
                     if (pi < b2) {
@@ -475,9 +497,12 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     }
                     const uint32_t g = off - prevoff;
                     // byte address 4*(8*(g/2) + row) = 16*g + 4*row (g even):
-                    // equal gaps of different rows fall in different banks
-                    if (g < 100u) atomicAdd((uint32_t*)(mqb + rowb + 16u * g), 1u);
-                    else pair_wide(pc, prevoff, g);
+                    // equal gaps of different rows fall in different banks.
+                    // Gaps >= 100 add into the unused bin 0 instead of taking
+                    // a branch around the atomic, and go to pair_wide.
+                    const uint32_t gn = g < 100u ? g : 0u;
+                    atomicAdd((uint32_t*)(mqb + rowb + 16u * gn), 1u);
+                    if (g >= 100u) pair_wide(pc, prevoff, g);
                     prevoff = off;
                     rowb = (b & 7u) << 2;
                 } while (w);
