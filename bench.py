@@ -42,7 +42,7 @@ LDS_PEAK_GBS = 128 * 256 * 2.4e9 / 1e9
 # runs (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, scaled by a 1 GiB copy kernel);
 # the commands and raw counters are in
 # profiles/sieve_kernel_plain_1e11_hbm_traffic.md.
-SIEVE_1E11_HBM_BYTES = (2 * 3897.375 + 17964.1875) * 1024
+SIEVE_1E11_HBM_BYTES = (2 * 3965.4 + 17968.3) * 1024
 
 
 def model_lds_bytes(N):
@@ -86,7 +86,46 @@ def measured_lds_peaks(copy_bytes=0):
         r = ctypes.c_double()
         if L.voss_probe_lds(k, 256, ctypes.byref(r)) == 0:
             out[name] = r.value / 1e9
+    L.voss_probe_montmul.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    r = ctypes.c_double()
+    if L.voss_probe_montmul(256, ctypes.byref(r)) == 0:
+        out["montmul_per_s"] = r.value
     return out
+
+
+MR_BASES = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37)
+
+
+def mr_mulmods(n):
+    """Algorithmic work of one Miller-Rabin test (is_prime_u64,
+    src/verify_cpp.cpp:38-57) in 64-bit modular multiplications: per base
+    tried, the square-and-multiply of mod_pow (:26-34: bitlen(d) squarings
+    and popcount(d) products) plus the squarings of the witness loop that
+    run; a composite stops at its first witness; trial division costs 0."""
+    if n < 2:
+        return 0
+    for b in MR_BASES:
+        if n == b or n % b == 0:
+            return 0
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        r += 1
+    per = d.bit_length() + bin(d).count("1")
+    work = 0
+    for a in MR_BASES:
+        work += per
+        x = pow(a, d, n)
+        if x == 1 or x == n - 1:
+            continue
+        for _ in range(r - 1):
+            x = x * x % n
+            work += 1
+            if x == n - 1:
+                break
+        else:
+            return work
+    return work
 
 
 def cpu_baseline(threads):
@@ -324,6 +363,18 @@ def main():
     assert np.array_equal(v_host, verdict)
     mr = {"tests_per_s": mr_rate, "count": int(len(cand)), "primes": int(verdict.sum()),
           "kernel_ms": mr_ms, "host_buffers_tests_per_s": len(cand) / t_host}
+    # roofline: algorithmic modular multiplications (exact on a sample of the
+    # same set, scaled) over the kernel time, against the measured rate of
+    # independent 64-bit Montgomery products on all CUs
+    n_sample = min(len(cand), 20000)
+    work = sum(mr_mulmods(int(x)) for x in cand[:n_sample]) * len(cand) / n_sample
+    mr_achieved = work / (statistics.median(mr_ms) / 1e3)
+    if probe and probe.get("montmul_per_s"):
+        mr["roofline"] = {"bound": "valu", "achieved": mr_achieved,
+                          "peak": probe["montmul_per_s"], "unit": "mulmod/s",
+                          "frac": mr_achieved / probe["montmul_per_s"],
+                          "mulmods_per_test": work / len(cand),
+                          "sample": f"exact count on the first {n_sample} candidates"}
     if not args.no_cpu:
         from oracle import oracle as O
         threads = min(16, os.cpu_count() or 1)

@@ -104,3 +104,14 @@ def test_sieve_cli_usage():
     r = subprocess.run([os.path.join(PKG, "bin", "voss_v7_live"), "1000", "31"],
                        capture_output=True, text=True)
     assert r.returncode == 2
+
+
+def test_bench_mr_work_count_matches_oracle(oracle):
+    """bench.py's restated Miller-Rabin work count equals the oracle's."""
+    import random
+    import bench
+    rng = random.Random(0x564F5353 + 31)
+    ns = [rng.randrange(2**40, 2**64) | 1 for _ in range(300)]
+    ns += [2**61 - 1, 3825123056546413051, 15, 2, 37, 1, 0, 341550071728321]
+    for n in ns:
+        assert bench.mr_mulmods(n) == oracle.mr_mulmods(n), n

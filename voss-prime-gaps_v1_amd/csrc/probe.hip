@@ -1,9 +1,13 @@
-// probe.hip -- LDS roofline microbenchmarks for gfx950 (used by bench.py to
-// state the measured peak beside the published one).  Every kernel keeps
-// its traffic inside one workgroup's LDS; the only global write is one
-// word per thread so the work cannot be optimised away.
+// probe.hip -- roofline microbenchmarks for gfx950 (used by bench.py to
+// state measured peaks beside the published ones): LDS bandwidth and
+// atomic rates, the HBM copy used to calibrate the traffic counters, and
+// the 64-bit Montgomery multiply rate that bounds Miller-Rabin.  The only
+// global write of the compute probes is one word per thread, so the work
+// cannot be optimised away.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "montgomery.h"
 
 namespace {
 
@@ -64,7 +68,50 @@ __global__ __launch_bounds__(256) void copy_b128(const uint4* __restrict__ src,
         dst[i] = src[i];
 }
 
+// Four independent chains of Montgomery products per lane, each lane with
+// its own odd 64-bit modulus (as in a Miller-Rabin batch): the latency of
+// one chain is hidden by the other three and by the other waves.
+__global__ __launch_bounds__(256) void montmul_chains(uint32_t iters, unsigned long long* out) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = (0x9E3779B97F4A7C15ull * (t + 1)) | 0x8000000000000001ull;
+    const voss::Mont M = voss::mont_setup(n);
+    uint64_t x0 = M.one, x1 = M.one + 1, x2 = M.one + 2, x3 = M.one + 3;
+    const uint64_t y = (0xD1B54A32D192ED03ull * (t + 7)) % n;
+    for (uint32_t i = 0; i < iters; i++) {
+        x0 = voss::mont_mul(x0, y, M);
+        x1 = voss::mont_mul(x1, y, M);
+        x2 = voss::mont_mul(x2, y, M);
+        x3 = voss::mont_mul(x3, y, M);
+    }
+    out[t] = x0 ^ x1 ^ x2 ^ x3;
+}
+
 }  // namespace
+
+// 64-bit Montgomery products per second on the whole device.
+extern "C" int voss_probe_montmul(int num_cus, double* rate) {
+    const uint32_t blocks = (uint32_t)num_cus * 8, threads = 256, iters = 4096;
+    unsigned long long* out = nullptr;
+    if (hipMalloc((void**)&out, (size_t)blocks * threads * 8) != hipSuccess) return -1;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    float best = 1e30f;
+    for (int rep = 0; rep < 4; rep++) {
+        (void)hipEventRecord(e0, 0);
+        hipLaunchKernelGGL(montmul_chains, dim3(blocks), dim3(threads), 0, 0, iters, out);
+        (void)hipEventRecord(e1, 0);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0 && ms < best) best = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    *rate = (double)blocks * threads * iters * 4.0 / (best * 1e-3);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 // One copy of `bytes` (multiple of 16) between two fresh device buffers.
 extern "C" int voss_probe_copy(size_t bytes, double* ms_out) {
