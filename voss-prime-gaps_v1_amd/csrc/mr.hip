@@ -19,15 +19,9 @@
 
 namespace voss {
 
-// Strong probable-prime test to the base whose Montgomery form is bm.
-__device__ __forceinline__ bool sprp(const Mont& M, uint64_t bm, uint64_t d,
-                                     int s) {
-    uint64_t x = bm;
-    const int top = 63 - __clzll((long long)d);
-    for (int k = top - 1; k >= 0; k--) {
-        x = mont_mul(x, x, M);
-        if ((d >> k) & 1) x = mont_mul(x, bm, M);
-    }
+// The witness loop of the strong probable-prime test after x = b^d: true
+// iff x = 1 or x reaches -1 within s - 1 squarings.
+__device__ __forceinline__ bool sprp_tail(const Mont& M, uint64_t x, int s) {
     const uint64_t mone = M.n - M.one;
     if (x == M.one || x == mone) return true;
     for (int j = 0; j < s - 1; j++) {
@@ -36,6 +30,45 @@ __device__ __forceinline__ bool sprp(const Mont& M, uint64_t bm, uint64_t d,
         if (x == M.one) return false;
     }
     return false;
+}
+
+// Base 2: left-to-right binary powering where "multiply by the base" is a
+// modular doubling, so the lanes whose exponent bit is set pay 4 integer
+// ops instead of a Montgomery product.
+__device__ __forceinline__ bool sprp2(const Mont& M, uint64_t d, int s) {
+    uint64_t x = add_mod(M.one, M.one, M.n);  // 2 in Montgomery form
+    const int top = 63 - __clzll((long long)d);
+    for (int k = top - 1; k >= 0; k--) {
+        x = mont_mul(x, x, M);
+        if ((d >> k) & 1) x = add_mod(x, x, M.n);
+    }
+    return sprp_tail(M, x, s);
+}
+
+// Other bases: fixed 3-bit windows, so every lane of a wave runs the same
+// squarings and products (a binary ladder runs a product at every bit where
+// any lane of the wave has a 1).  ktop = highest digit index any lane of
+// the wave needs (d < 2^63: at most 20); tbl holds b^0..b^7 in Montgomery
+// form, 8 rows of the block's 256 lanes.
+__device__ __forceinline__ bool sprp_window(const Mont& M, uint64_t bm, uint64_t d, int s,
+                                            int ktop, uint64_t* tbl) {
+    const uint32_t t = threadIdx.x;
+    tbl[t] = M.one;
+    tbl[256 + t] = bm;
+    uint64_t p = bm;
+#pragma unroll
+    for (int k = 2; k < 8; k++) {
+        p = mont_mul(p, bm, M);
+        tbl[k * 256 + t] = p;
+    }
+    uint64_t x = tbl[((uint32_t)(d >> (3 * ktop)) & 7u) * 256 + t];
+    for (int k = ktop - 1; k >= 0; k--) {
+        x = mont_mul(x, x, M);
+        x = mont_mul(x, x, M);
+        x = mont_mul(x, x, M);
+        x = mont_mul(x, tbl[((uint32_t)(d >> (3 * k)) & 7u) * 256 + t], M);
+    }
+    return sprp_tail(M, x, s);
 }
 
 __constant__ uint32_t c_bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
@@ -71,8 +104,7 @@ __global__ __launch_bounds__(256) void mr_stage1(const uint64_t* __restrict__ ca
                 uint64_t d = n - 1;
                 const int s = __ffsll((long long)d) - 1;
                 d >>= s;
-                const uint64_t b2 = add_mod(M.one, M.one, n);
-                v = sprp(M, b2, d, s) ? 2 : 0;
+                v = sprp2(M, d, s) ? 2 : 0;
             }
             more = (v == 2);
             verdict[i] = (uint8_t)(v == 1);
@@ -93,22 +125,38 @@ __global__ __launch_bounds__(256) void mr_stage2(const uint64_t* __restrict__ ca
                                                  const uint32_t* __restrict__ surv,
                                                  const uint32_t* __restrict__ nsurv,
                                                  uint8_t* __restrict__ verdict) {
+    __shared__ uint64_t tbl[8 * 256];
     const uint32_t ns = *nsurv;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += stride) {
-        const uint32_t i = surv[k];
-        const uint64_t n = cand[i];
+    // the loop bound is uniform over the block, so the whole wave is present
+    // for the digit-count reduction below
+    for (uint32_t base = blockIdx.x * blockDim.x; base < ns; base += stride) {
+        const uint32_t k = base + threadIdx.x;
+        const bool active = k < ns;
+        uint64_t n = 3, d = 1;
+        int s = 1;
+        uint32_t i = 0;
+        if (active) {
+            i = surv[k];
+            n = cand[i];
+            d = n - 1;
+            s = __ffsll((long long)d) - 1;
+            d >>= s;
+        }
+        // highest base-8 digit of d that any lane of this wave needs (d < 2^63)
+        int ktop = active ? (63 - __clzll((long long)d)) / 3 : 0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) ktop = max(ktop, __shfl_xor(ktop, o));
+        ktop = min(ktop, 20);
+        if (!active) continue;
         const Mont M = mont_setup(n);
-        uint64_t d = n - 1;
-        const int s = __ffsll((long long)d) - 1;
-        d >>= s;
         uint64_t bm = add_mod(M.one, M.one, n);  // base 2
         uint32_t b = 2;
         bool prime = true;
         for (int t = 1; t < 12 && prime; t++) {
             const uint32_t nb = c_bases[t];
             for (; b < nb; b++) bm = add_mod(bm, M.one, n);
-            prime = sprp(M, bm, d, s);
+            prime = sprp_window(M, bm, d, s, ktop, tbl);
         }
         verdict[i] = prime ? 1 : 0;
     }
