@@ -1311,6 +1311,9 @@ Plan plain_plan(const voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t s0, ui
     Plan P;
     P.njobs = 1;
     const uint32_t W = c->tune.window_bytes;
+    // windows of the range plus one split per logical-segment edge
+    P.desc.reserve((size_t)((std::min<uint64_t>(s1 * seg_num, N) - std::min<uint64_t>(s0 * seg_num, N)) /
+                            (30ull * W)) + 2 * (size_t)(s1 - s0) + 2);
     for (uint64_t s = s0; s < s1; s++) {
         const uint64_t a = s * seg_num + 1;
         const uint64_t b = std::min<uint64_t>((s + 1) * seg_num, N);

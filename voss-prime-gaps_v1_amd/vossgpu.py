@@ -204,10 +204,10 @@ def result_to_dict(r, n_large=None):
         "last_prime": int(r.last_prime),
         "c1": int(r.c1),
         "c3": int(r.c3),
-        "modq": np.ctypeslib.as_array(r.modq).astype(np.uint64).copy(),
-        "gaps": np.ctypeslib.as_array(r.gaps).astype(np.int64).copy(),
-        "modq_gap": np.ctypeslib.as_array(r.modq_gap).astype(np.int64)
-                      .reshape(MODQ_ROWS, MODQ_MAX_GAP).copy(),
+        "modq": np.array(np.ctypeslib.as_array(r.modq), dtype=np.uint64),
+        "gaps": np.array(np.ctypeslib.as_array(r.gaps), dtype=np.int64),
+        "modq_gap": np.array(np.ctypeslib.as_array(r.modq_gap), dtype=np.int64)
+                      .reshape(MODQ_ROWS, MODQ_MAX_GAP),
         "large_count": int(r.large_count),
         "large": large_array(r.large, n),
         "seconds": float(r.seconds),
