@@ -276,12 +276,13 @@ int build_patterns(voss_ctx* c) {
                 for (uint64_t b = b0; b < bytes.size(); b += p) bytes[b] &= (uint8_t)~(1u << o);
             }
         }
-        for (uint32_t w = 0; w < P; w++) {
+        for (uint32_t w = 0; w <= P; w++) {  // word P repeats word 0
             uint32_t v;
-            std::memcpy(&v, &bytes[4 * (size_t)w], 4);
+            std::memcpy(&v, &bytes[4 * (size_t)(w % P)], 4);
             all.push_back(v);
         }
     }
+    if (all.size() != (size_t)kPresieveWordsPadded) return fail(VOSS_EDEVICE, "presieve table size");
     // word 0 (numbers 1..119): put back the presieved primes themselves
     uint32_t restore = 0;
     for (int g = 0; g < kPresieveGroups; g++)
@@ -501,7 +502,7 @@ SieveArgs make_args(voss_ctx* c, ExecBufs& B, uint32_t ndesc, unsigned long long
     for (int g = 0; g < kPresieveGroups; g++) {
         a.pat_off[g] = c->pat_off[g];
         a.pat_len[g] = c->pat_len[g];
-        a.pat_step[g] = (uint32_t)block % c->pat_len[g];
+        a.pat_step[g] = (2u * (uint32_t)block) % c->pat_len[g];  // a thread's next word pair
     }
     a.lg_thresh = c->d_lg_thresh.p;
     a.bucket_off = B.d_boff.p;

@@ -197,10 +197,10 @@ __device__ __forceinline__ void pair_wide(const PairCtx& c, uint32_t pb, uint32_
 template <int WIN, int BLOCK, bool EMIT>
 __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
     constexpr int W4 = WIN / 4;
-    __shared__ uint32_t bm[W4];
+    __shared__ __align__(16) uint32_t bm[W4];
     __shared__ uint32_t hist[kHistBins];
     __shared__ uint32_t mq[64 * 8];  // q=30 rows interleaved: mq[8*(g/2) + row]
-    __shared__ uint32_t s_pat[kPresieveWords];
+    __shared__ uint32_t s_pat[kPresieveWordsPadded];
     __shared__ uint32_t tfirst[BLOCK];
     __shared__ uint32_t tlast[BLOCK];
     __shared__ unsigned long long s_cnt[10];
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
 
     for (int i = tid; i < kHistBins; i += BLOCK) hist[i] = 0;
     for (int i = tid; i < 8 * 64; i += BLOCK) mq[i] = 0;
-    for (int i = tid; i < kPresieveWords; i += BLOCK) s_pat[i] = a.pat[i];
+    for (int i = tid; i < kPresieveWordsPadded; i += BLOCK) s_pat[i] = a.pat[i];
     for (int i = tid; i < kLgtCount; i += BLOCK) s_lgt[i] = a.lg_thresh[kLgtFirst + i];
     uint32_t cur_job = blockIdx.x < a.ndesc ? a.desc[blockIdx.x].job : 0;
 
@@ -266,20 +266,25 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
 
         // ---- phase 0: presieve patterns (primes 7..73) from LDS tables
         {
+            // each thread fills word pairs (2t, 2t+1), (2t+2B, 2t+2B+1), ...:
+            // a pair reads two consecutive pattern words per table (the
+            // padding word makes the second read safe at the table's end)
             uint32_t idx[kPresieveGroups];
 #pragma unroll
             for (int g = 0; g < kPresieveGroups; g++)
-                idx[g] = (s_patbase[g] + (uint32_t)tid) % a.pat_len[g];
-            for (uint32_t i = tid; i < L4; i += BLOCK) {
-                uint32_t w = 0xFFFFFFFFu;
+                idx[g] = (s_patbase[g] + 2u * (uint32_t)tid) % a.pat_len[g];
+            for (uint32_t i = 2u * tid; i < L4; i += 2u * BLOCK) {
+                uint32_t w0 = 0xFFFFFFFFu, w1 = 0xFFFFFFFFu;
 #pragma unroll
                 for (int g = 0; g < kPresieveGroups; g++) {
-                    w &= s_pat[a.pat_off[g] + idx[g]];
+                    const uint32_t* t = s_pat + a.pat_off[g] + idx[g];
+                    w0 &= t[0];
+                    w1 &= t[1];
                     const uint32_t v = idx[g] + a.pat_step[g];
                     idx[g] = v >= a.pat_len[g] ? v - a.pat_len[g] : v;
                 }
-                if (i == 0 && D.w0 == 0) w = (w | a.restore0) & ~1u;
-                bm[i] = w;
+                if (i == 0 && D.w0 == 0) w0 = (w0 | a.restore0) & ~1u;
+                *reinterpret_cast<uint2*>(&bm[i]) = make_uint2(w0, w1);
             }
         }
         __syncthreads();

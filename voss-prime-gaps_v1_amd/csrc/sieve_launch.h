@@ -13,7 +13,7 @@ struct SieveArgs {
     uint32_t tier_a_end;    // first prime index marked one-lane-per-prime
     uint32_t restore0;      // word 0 bits of the presieved primes 7..73
     const PrimeRec* primes; // base primes >= 7, ascending
-    const uint32_t* pat;                 // all tables, kPresieveWords words
+    const uint32_t* pat;                 // all tables, kPresieveWordsPadded words
     uint32_t pat_off[kPresieveGroups];   // table start in pat
     uint32_t pat_len[kPresieveGroups];   // period in words
     uint32_t pat_step[kPresieveGroups];  // block size mod period

@@ -34,6 +34,9 @@ constexpr int kPresieveGroupPrimes[kPresieveGroups][3] = {
     {67, 1, 1},  {71, 1, 1},  {73, 1, 1}};
 constexpr int kPresieveWords = 1001 + 323 + 667 + 1147 + 41 + 43 + 47 + 53 +
                                59 + 61 + 67 + 71 + 73;
+// Each table is followed by a copy of its first word, so two consecutive
+// pattern words can be read without a wrap check.
+constexpr int kPresieveWordsPadded = kPresieveWords + kPresieveGroups;
 
 // One unit of sieve work: an LDS window of `len` wheel bytes starting at
 // byte w0 (w0 % 4 == 0, len % 4096 == 0), of which the numbers in
