@@ -284,7 +284,7 @@ def serve_leg(eng, W, n_req=300, n_cal=80):
 def interval_leg(eng):
     """Config 3: [1e13, 1e13 + 1e12), base primes to 3.32e6."""
     lo, hi = 10**13, 10**13 + 10**12
-    eng.run_interval(lo, lo + 10**9)
+    eng.run_interval(lo, hi, large_cap=1 << 22)  # warm-up: the same call
     t = time.perf_counter()
     r = eng.run_interval(lo, hi, large_cap=1 << 22)
     dt = time.perf_counter() - t

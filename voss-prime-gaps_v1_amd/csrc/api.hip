@@ -11,6 +11,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mr_launch.h"
@@ -164,6 +165,24 @@ bool merit_ge10(uint64_t p_before, uint32_t g) {
     return (double)g / std::log((double)p_before) >= 10.0;
 }
 
+// f(begin, end) over [0, n) split across up to 16 host threads when n is
+// large enough to pay for starting them.
+template <typename F>
+void parallel_for(size_t n, F&& f) {
+    const size_t kMinPerThread = 32768;
+    size_t nt = std::min<size_t>(16, std::max<unsigned>(1u, std::thread::hardware_concurrency()));
+    nt = std::min(nt, std::max<size_t>(1, n / kMinPerThread));
+    if (nt <= 1) {
+        f((size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++)
+        th.emplace_back([&, t] { f(n * t / nt, n * (t + 1) / nt); });
+    f((size_t)0, n / nt);
+    for (auto& x : th) x.join();
+}
+
 }  // namespace
 
 struct voss_ctx {
@@ -173,7 +192,7 @@ struct voss_ctx {
     std::string name;
     int num_cus = 0;
     double total_gb = 0;
-    voss_tuning tune{131072, 1, 1024, 1u << 21, 2, 100, {0, 0}};
+    voss_tuning tune{131072, 1, 768, 1u << 21, 2, 100, {0, 0}};
 
     std::vector<uint32_t> primes;
     std::vector<double> inv_prefix;  // inv_prefix[i] = sum of 1/p over primes[0..i)
@@ -969,10 +988,15 @@ struct Executor {
             if (B->h_acc.p[j].error) return fail(VOSS_ERANGE, "prime gap >= 2048 (beyond the device histogram)");
             add_acc(jobs[j], B->h_acc.p[j]);
         }
+        // merits (one log each) on several host threads for long lists
+        std::vector<float> merit(nlarge);
+        parallel_for(nlarge, [&](size_t k0, size_t k1) {
+            for (size_t k = k0; k < k1; k++)
+                merit[k] = merit_of(B->h_large.p[k].p_before, B->h_large.p[k].gap);
+        });
         for (uint64_t k = 0; k < nlarge; k++) {
             const LargeRaw& r = B->h_large.p[k];
-            jobs[r.job].large.push_back({r.p_before + r.gap, (int32_t)r.gap,
-                                         merit_of(r.p_before, r.gap)});
+            jobs[r.job].large.push_back({r.p_before + r.gap, (int32_t)r.gap, merit[k]});
         }
         const SegRec* rec = B->h_rec.p;
         if (counts) {
@@ -1246,7 +1270,7 @@ int voss_autotune(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
     // window, two 64 KiB or four 32 KiB ones
     const std::vector<std::pair<uint32_t, uint32_t>> shapes = {
         {131072, 1}, {65536, 2}, {65536, 1}, {32768, 4}};
-    const std::vector<uint32_t> tiers = {512, 1024, 2048};
+    const std::vector<uint32_t> tiers = {512, 768, 1024, 2048};
     const std::vector<uint32_t> blimits = {1u << 17, 1u << 19, 1u << 21, 1u << 23, 1u << 25,
                                            0xFFFFFFFFu};
     const std::vector<uint32_t> bmins = {2, 4, 8};
