@@ -73,12 +73,15 @@ typedef struct {
     double seconds;          /* out: wall time from planning to the final
                                 histogram on the host (src/voss_v7_live.cu:
                                 432 -> 632 window) */
-    double device_ms;        /* out: summed device time of the sieve and
-                                stitch kernels (HIP events) */
-    double sieve_ms;         /* out: device time of the sieve kernel alone */
+    double device_ms;        /* out: device time of the whole run: uploads,
+                                bucket fills, sieves, stitches, read-backs
+                                (HIP events) */
+    double sieve_ms;         /* out: device time of the sieve kernels alone
+                                (-1 when the run was one captured graph) */
     uint64_t windows;        /* out: LDS windows sieved */
-    double bucket_ms;        /* out: device time of the large-prime bucket passes */
-    uint64_t bucket_entries; /* out: bucket entries written (large-prime hits) */
+    double bucket_ms;        /* out: device time of the large-prime bucket
+                                fills (-1 under a graph) */
+    uint64_t bucket_entries; /* out: bucket room reserved for large-prime hits */
 } voss_result;
 
 /* Compact result of one interval of a batch.  gaps[h] counts the gap 2h

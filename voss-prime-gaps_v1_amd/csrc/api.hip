@@ -1004,7 +1004,9 @@ struct Executor {
             for (size_t i = 0; i < P.desc.size(); i++) (*counts)[i] = rec[i].count;
         }
         // round edge: pair the last prime of a job before the round with the
-        // first non-empty window of the same job in the round
+        // first non-empty window of the same job in the round (nothing to do
+        // for a single round: the stitch kernel joined all its windows)
+        if (rounds.size() < 2) return VOSS_OK;
         std::vector<uint64_t> carry_last(P.njobs, 0);
         std::vector<uint32_t> carry_lseg(P.njobs, 0);
         std::vector<uint8_t> carry_has(P.njobs, 0), seen(P.njobs, 0);
@@ -1055,10 +1057,18 @@ int check_result(voss_result* out) {
     return VOSS_OK;
 }
 
-void clear_result(voss_result* out) {
+// Zeroes every field but the caller's large-gap buffer.  The two big
+// histograms are left to write_result, which fills them completely.
+void clear_result(voss_result* out, bool histograms = true) {
     voss_large_gap* lg = out->large;
     const uint64_t cap = out->large_capacity;
-    std::memset(out, 0, sizeof(*out));
+    if (histograms) {
+        std::memset(out, 0, sizeof(*out));
+    } else {
+        std::memset(out, 0, offsetof(voss_result, gaps));
+        std::memset(reinterpret_cast<char*>(out) + offsetof(voss_result, large_count), 0,
+                    sizeof(*out) - offsetof(voss_result, large_count));
+    }
     out->large = lg;
     out->large_capacity = cap;
 }
@@ -1097,7 +1107,7 @@ IntervalHead finish_interval_common(JobHost& J, uint64_t lo, uint64_t hi) {
 }
 
 void finish_interval(JobHost& J, uint64_t lo, uint64_t hi, voss_result* out) {
-    clear_result(out);
+    clear_result(out, false);  // write_result fills the histograms
     const IntervalHead H = finish_interval_common(J, lo, hi);
     out->total_primes = H.total;
     out->first_prime = H.first;
@@ -1463,7 +1473,7 @@ int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) 
     uint64_t nwin = 0;
     const uint64_t num_seg = (N + seg_num - 1) / seg_num;  // :342
     if ((rc = plain_range(c, N, seg_num, 0, num_seg, out, J, tm, &nwin))) return rc;
-    clear_result(out);
+    clear_result(out, false);  // write_result fills the histograms
     // (5, first wheel prime) is the reference's host boundary gap of
     // segment 0 (:559-562): main histogram only
     if (J.count) host_pair(J, 5, J.first, false);
@@ -1494,7 +1504,7 @@ int voss_run_plain_range(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t seg
     uint64_t nwin = 0;
     if ((rc = plain_range(c, N, seg_num, seg_begin, seg_end, out, J, tm, &nwin))) return rc;
     if (J.overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");
-    clear_result(out);
+    clear_result(out, false);  // write_result fills the histograms
     out->total_primes = J.count;
     out->first_prime = J.count ? J.first : 0;
     out->last_prime = J.count ? J.last : 0;

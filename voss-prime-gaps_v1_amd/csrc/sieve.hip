@@ -483,23 +483,18 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                 prev = cnt ? 1u : 0u;
             }
             char* const mqb = (char*)mq;
-            for (; i < iend; i++) {
-                uint32_t w = bm[i];
-                if (!w) continue;
-                const uint32_t base = 120u * i;
-                do {
-                    const uint32_t b = __builtin_ctz(w);
-                    w &= w - 1u;
-                    // 30*(b>>3) + W30[b&7]; v_perm picks byte (b&7) of W30 and
-                    // selector bytes 0x0C give zeros above it
-                    const uint32_t w30 = __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u,
-                                                               (b & 7u) | 0x0C0C0C00u);
-                    const uint32_t off = base + 30u * (b >> 3) + w30;
-                    if constexpr (EMIT) {
-                        a.emit_out[emit_at++] = base_num + off;
-                        prevoff = off;
-                        continue;
-                    }
+            // one prime: the lowest set bit of w (cleared), in word base
+            auto visit = [&](uint32_t& w, uint32_t base) {
+                const uint32_t b = __builtin_ctz(w);
+                w &= w - 1u;
+                // 30*(b>>3) + W30[b&7]; v_perm picks byte (b&7) of W30 and
+                // selector bytes 0x0C give zeros above it
+                const uint32_t w30 = __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u,
+                                                           (b & 7u) | 0x0C0C0C00u);
+                const uint32_t off = base + 30u * (b >> 3) + w30;
+                if constexpr (EMIT) {
+                    a.emit_out[emit_at++] = base_num + off;
+                } else {
                     const uint32_t g = off - prevoff;
                     // byte address 4*(8*(g/2) + row) = 16*g + 4*row (g even):
                     // equal gaps of different rows fall in different banks.
@@ -508,9 +503,16 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     const uint32_t gn = g < 100u ? g : 0u;
                     atomicAdd((uint32_t*)(mqb + rowb + 16u * gn), 1u);
                     if (g >= 100u) pair_wide(pc, prevoff, g);
-                    prevoff = off;
                     rowb = (b & 7u) << 2;
-                } while (w);
+                }
+                prevoff = off;
+            };
+            for (; i < iend; i++) {
+                uint32_t w = bm[i];
+                if (!w) continue;
+                const uint32_t base = 120u * i;
+                do visit(w, base);
+                while (w);
             }
             prev = prev ? prevoff + 1u : 0u;
         }
