@@ -91,7 +91,8 @@ struct ExecBufs {
     DevBuf<SegRec> d_rec;
     DevBuf<JobAcc> d_acc;
     DevBuf<LargeRaw> d_large;
-    DevBuf<unsigned long long> d_flags;  // [0] large-gap records, [1] bucket overflow
+    DevBuf<unsigned long long> d_flags;  // [0] large-gap records, [1] bucket overflow,
+                                         // [2] bucket task counter
     DevBuf<BucketTask> d_btask;
     DevBuf<uint32_t> d_bcnt;
     DevBuf<uint32_t> d_bcur;
@@ -550,9 +551,6 @@ struct ExecTimes {
 
 // Bucket entries per round (8 GiB of HBM); offsets stay 32-bit.
 constexpr uint64_t kBucketEntryBudget = 1ull << 31;
-// Primes per bucket task: long tiles give every thread many hits already;
-// short tiles need more primes per task to cover the task's fixed costs.
-uint32_t bucket_chunk(uint32_t nwin) { return nwin >= 256 ? 512u : (nwin >= 32 ? 2048u : 8192u); }
 
 // Room for the entries of one bucket (a window of W bytes) fed by primes
 // with sum(1/p) = s.  A prime p has on average 8W/p multiples coprime to 30
@@ -578,6 +576,16 @@ struct Round {
     size_t task_off = 0, bkt_off = 0, boff_off = 0;  // places in the staged arrays
     uint64_t room_total = 0;
 };
+
+// Estimated VALU work of a bucket task: per-item setup (8 items per prime
+// in class mode, 1 in walk mode), class-mode hits, and walk steps (30
+// span/p per walked prime).
+double bucket_task_cost(const voss_ctx* c, const BucketTask& T, uint32_t W) {
+    const double span = (double)T.nwin * W;
+    return 25.0 * (8.0 * (double)(T.iw - T.i0) + (double)(T.i1 - T.iw)) +
+           40.0 * span * (c->inv_prefix[T.iw] - c->inv_prefix[T.i0]) +
+           210.0 * span * (c->inv_prefix[T.i1] - c->inv_prefix[T.iw]);
+}
 
 void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
     const uint32_t W = c->tune.window_bytes;
@@ -608,6 +616,7 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
         }
         R.nd = end - off;
         R.desc.assign(P.desc.begin() + off, P.desc.begin() + end);
+        std::vector<std::pair<double, BucketTask>> work;  // tasks with their estimated cost
         // buckets: consecutive windows of one job with the same w0 share one
         size_t i = 0;
         while (i < R.nd) {
@@ -618,15 +627,18 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
             uint32_t nb0 = R.nbuckets;
             uint64_t prev_w0 = ~0ull;
             uint32_t np_max = 0;
+            std::vector<uint32_t> np_of;  // per bucket of the run: primes its windows need
             while (j < R.nd && R.desc[j].job == d.job && bucketed(R.desc[j])) {
                 SegDesc& e = R.desc[j];
                 if (e.w0 != prev_w0) {
                     if (prev_w0 != ~0ull && e.w0 != prev_w0 + W) break;
                     R.nbuckets++;
                     prev_w0 = e.w0;
+                    np_of.push_back(0);
                 }
                 e.bucket = R.nbuckets - 1;
                 np_max = std::max(np_max, e.nprimes);
+                np_of.back() = std::max(np_of.back(), e.nprimes);
                 e.nprimes = (uint32_t)ibl;
                 j++;
             }
@@ -637,31 +649,59 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
             R.room_total += (uint64_t)room * nwin;
             for (uint32_t t0 = 0; t0 < nwin; t0 += kBucketTileMax) {
                 const uint32_t tn = std::min<uint32_t>(kBucketTileMax, nwin - t0);
+                const uint32_t np_t = *std::max_element(np_of.begin() + t0, np_of.begin() + t0 + tn);
                 // primes beyond 1.5x the tile length: walk mode
                 const uint64_t walk_from = std::min<uint64_t>(3ull * tn * W / 2, 0xFFFFFFFFu);
                 const size_t iw = std::min<size_t>(
-                    np_max, std::max<size_t>(ibl, (size_t)(std::lower_bound(
-                                                               c->primes.begin(), c->primes.end(),
-                                                               (uint32_t)walk_from) -
-                                                           c->primes.begin())));
-                for (int walk = 0; walk < 2; walk++) {
-                    const size_t a0 = walk ? iw : ibl, a1 = walk ? np_max : iw;
-                    const uint32_t chunk = bucket_chunk(tn) * (walk ? 8u : 1u);
-                    for (size_t p0 = a0; p0 < a1; p0 += chunk) {
-                        BucketTask T;
-                        T.w0 = d.w0 + (uint64_t)t0 * W;
-                        T.b0 = nb0 + t0;
-                        T.nwin = tn;
-                        T.i0 = (uint32_t)p0;
-                        T.i1 = (uint32_t)std::min<size_t>(a1, p0 + chunk);
-                        T.walk = (uint32_t)walk;
-                        T.pad = 0;
-                        R.tasks.push_back(T);
-                    }
-                }
+                    np_t, std::max<size_t>(ibl, (size_t)(std::lower_bound(
+                                                             c->primes.begin(), c->primes.end(),
+                                                             (uint32_t)walk_from) -
+                                                         c->primes.begin())));
+                BucketTask T;
+                T.w0 = d.w0 + (uint64_t)t0 * W;
+                T.b0 = nb0 + t0;
+                T.nwin = tn;
+                T.i0 = (uint32_t)ibl;
+                T.iw = (uint32_t)iw;
+                T.i1 = np_t;
+                T.shared = 0;
+                work.push_back({bucket_task_cost(c, T, W), T});
             }
             i = j;
         }
+        // a tile with more work than a workgroup's fair share is split into
+        // prime chunks of about a quarter of that share (shared tasks)
+        double total = 0;
+        for (const auto& w : work) total += w.first;
+        const double share = total / (2.0 * std::max(c->num_cus, 1));  // two workgroups per CU
+        std::vector<std::pair<double, BucketTask>> split;
+        for (const auto& w : work) {
+            if (w.first <= share) {
+                split.push_back(w);
+                continue;
+            }
+            const BucketTask& T = w.second;
+            const uint32_t k = (uint32_t)std::min(256.0, std::ceil(4.0 * w.first / share));
+            for (int walk = 0; walk < 2; walk++) {
+                const uint32_t a0 = walk ? T.iw : T.i0, a1 = walk ? T.i1 : T.iw;
+                if (a1 <= a0) continue;
+                const uint32_t step = std::max<uint32_t>(1, (a1 - a0 + k - 1) / k);
+                for (uint32_t x = a0; x < a1; x += step) {
+                    BucketTask S = T;
+                    S.i0 = x;
+                    S.iw = walk ? x : std::min(a1, x + step);
+                    S.i1 = std::min(a1, x + step);
+                    S.shared = 1;
+                    split.push_back({bucket_task_cost(c, S, W), S});
+                }
+            }
+        }
+        work.swap(split);
+        // largest tasks first, so the last ones to start are the short ones
+        std::stable_sort(work.begin(), work.end(),
+                         [](const std::pair<double, BucketTask>& x,
+                            const std::pair<double, BucketTask>& y) { return x.first > y.first; });
+        for (const auto& w : work) R.tasks.push_back(w.second);
         rounds.push_back(std::move(R));
         off = end;
     }
@@ -743,7 +783,7 @@ struct Executor {
         HIPCHK(B->d_rec.reserve(nd));
         HIPCHK(B->d_acc.reserve(nj));
         HIPCHK(B->d_large.reserve(lcap));
-        HIPCHK(B->d_flags.reserve(2));
+        HIPCHK(B->d_flags.reserve(3));
         HIPCHK(B->d_btask.reserve(ntask));
         HIPCHK(B->d_bcnt.reserve(nb_max));
         HIPCHK(B->d_bcur.reserve(nbkt));
@@ -773,6 +813,8 @@ struct Executor {
     }
 
     hipEvent_t ev(size_t i) { return c->evs[i]; }
+
+    uint32_t* task_counter() { return reinterpret_cast<uint32_t*>(B->d_flags.p + 2); }
 
     int ensure_events(size_t n) {
         while (c->evs.size() < n) {
@@ -813,11 +855,14 @@ struct Executor {
         for (size_t r = 0; r < rounds.size(); r++) {
             const Round& R = rounds[r];
             if (timed) HIPCHK(hipEventRecord(ev(4 * r), s));
-            if (!R.tasks.empty())
+            if (!R.tasks.empty()) {
+                HIPCHK(launch_fill_u32(task_counter(), 1, 0, s));
                 HIPCHK(launch_bucket_fill(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
                                           c->d_primes.p, B->d_boff.p + R.boff_off,
                                           B->d_bcur.p + R.bkt_off, B->d_bent.p,
-                                          (uint32_t*)(B->d_flags.p + 1), log2w, bgrid, s));
+                                          (uint32_t*)(B->d_flags.p + 1), task_counter(), log2w,
+                                          bgrid, s));
+            }
             if (timed) HIPCHK(hipEventRecord(ev(4 * r + 1), s));
             const SieveArgs a = round_args(R);
             HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, s));
@@ -860,8 +905,10 @@ struct Executor {
             if (timed) HIPCHK(hipEventRecord(ev(4 * r), s));
             if (!R.tasks.empty()) {
                 HIPCHK(launch_fill_u32(B->d_bcnt.p, R.nbuckets, 0, s));
+                HIPCHK(launch_fill_u32(task_counter(), 1, 0, s));
                 HIPCHK(launch_bucket_count(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
-                                           c->d_primes.p, B->d_bcnt.p, log2w, bgrid, s));
+                                           c->d_primes.p, B->d_bcnt.p, task_counter(), log2w,
+                                           bgrid, s));
                 HIPCHK(hipMemcpyAsync(B->h_bcnt.p, B->d_bcnt.p, R.nbuckets * 4,
                                       hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
@@ -879,10 +926,12 @@ struct Executor {
                 HIPCHK(hipMemcpyAsync(B->d_boff.p + R.boff_off, B->h_boff.p + R.boff_off,
                                       (R.nbuckets + 1) * 8, hipMemcpyHostToDevice, s));
                 HIPCHK(launch_fill_u32(B->d_bcur.p + R.bkt_off, R.nbuckets, 0, s));
+                HIPCHK(launch_fill_u32(task_counter(), 1, 0, s));
                 HIPCHK(launch_bucket_fill(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
                                           c->d_primes.p, B->d_boff.p + R.boff_off,
                                           B->d_bcur.p + R.bkt_off, B->d_bent.p,
-                                          (uint32_t*)(B->d_flags.p + 1), log2w, bgrid, s));
+                                          (uint32_t*)(B->d_flags.p + 1), task_counter(), log2w,
+                                          bgrid, s));
             }
             if (timed) HIPCHK(hipEventRecord(ev(4 * r + 1), s));
             const SieveArgs a = round_args(R);
@@ -1711,7 +1760,7 @@ int voss_sample_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, int32_t* gap_se
         HIPCHK(c->d_emit.reserve(std::max<unsigned long long>(total, 1)));
         HIPCHK(B.d_acc.reserve(1));
         HIPCHK(B.d_large.reserve(1));
-        HIPCHK(B.d_flags.reserve(2));
+        HIPCHK(B.d_flags.reserve(3));
         HIPCHK(hipMemcpyAsync(B.d_desc.p, emit_desc.data(), nd * sizeof(SegDesc),
                               hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->d_emit_off.p, emit_off.data(), nd * 8, hipMemcpyHostToDevice,

@@ -625,8 +625,14 @@ __global__ void init_acc_kernel(JobAcc* acc, uint32_t njobs) {
     if (threadIdx.x == 0) a->first = ~0ull;
 }
 
-// Large-prime buckets.  One workgroup per task (a tile of consecutive
-// windows of one job and a range of primes).  Two ways to enumerate the
+// Large-prime buckets.  One workgroup per task: a tile of <= 256
+// consecutive windows of one job and the primes >= bucket_limit of the
+// task.  An owner task has every prime of its tile, so the tile's buckets
+// have a single writer and the slots are handed out by LDS atomics alone;
+// a shared task (a prime chunk of a tile too big for one workgroup) counts
+// its hits per window in LDS first and reserves them with one global atomic
+// per window on the bucket's cursor (the cursors start at 0).  Tasks are
+// taken from a device counter, largest first (the host orders them).  Two ways to enumerate the
 // composite multiples m*p (m >= 2, coprime to 30) of a prime in the tile:
 //   class mode  thread per (prime, residue class): first multiple of the
 //               class by Barrett reduction, then stride p bytes.  Best when
@@ -636,22 +642,23 @@ __global__ void init_acc_kernel(JobAcc* acc, uint32_t njobs) {
 //               m -> m+1 (p/30 bytes and p%30 in the residue) and keeps the
 //               multiples coprime to 30.  Best for p beyond the tile length,
 //               where a class mode thread mostly finds nothing.
-// COUNT adds the hits per window into bucket_cnt.  FILL counts them in
-// LDS, reserves each window's slots with one global atomic on its cursor,
-// and writes the entries (offset << 3) | bit.  A bucket holds
-// bucket_off[b+1] - bucket_off[b] entries: a reservation beyond that
-// writes nothing and sets *overflow (the caller then re-runs with exact
-// counts), so no write ever leaves the bucket.
+// COUNT writes the hits per window to bucket_cnt.  FILL starts each
+// window's LDS cursor at its bucket's first slot and writes the entries
+// (offset << 3) | bit; a bucket holds bucket_off[b+1] - bucket_off[b]
+// entries: hits beyond that are not written, the window's count (written to
+// bucket_cnt) still includes them and *overflow is set (the caller then
+// re-runs with exact counts), so no write ever leaves the bucket.
 
 // Calls f(byte offset in the tile, wheel bit) for every hit of work item k
-// (record pr already loaded: prime k in walk mode, prime k/8 otherwise).
+// of a tile starting at byte w0 (record pr already loaded: prime k in walk
+// mode, prime k/8 otherwise).
 template <typename F>
-__device__ __forceinline__ void bucket_hits(const BucketTask& T, const PrimeRec& pr, uint32_t k,
+__device__ __forceinline__ void bucket_hits(bool walk, uint64_t w0, const PrimeRec& pr, uint32_t k,
                                             uint32_t span, F&& f) {
-    if (T.walk) {
+    if (walk) {
         const uint32_t p = pr.p;
-        const uint64_t X = 30ull * T.w0;  // first number of the tile
-        uint32_t x;                       // first multiple m*p >= X with m >= 2, minus X
+        const uint64_t X = 30ull * w0;  // first number of the tile
+        uint32_t x;                     // first multiple m*p >= X with m >= 2, minus X
         if (X < 2ull * p) {
             x = (uint32_t)(2ull * p - X);
         } else {
@@ -672,24 +679,24 @@ __device__ __forceinline__ void bucket_hits(const BucketTask& T, const PrimeRec&
     } else {
         const int j = (int)(k & 7u);
         const uint32_t p = pr.p;
-        const uint32_t r = mod_u64(T.w0, p, pr.magic);
+        const uint32_t r = mod_u64(w0, p, pr.magic);
         const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
-        for (uint32_t b = first_pos(p, r, j, T.w0); b < span; b += p) f(b, bit);
+        for (uint32_t b = first_pos(p, r, j, w0); b < span; b += p) f(b, bit);
     }
 }
 
 // FILL in class mode, four hits per step: the four slot reservations (LDS
 // atomics with return) and the four room reads are issued before any result
 // is used, so their latencies overlap instead of adding up per hit.
-__device__ __forceinline__ void bucket_fill_class(const BucketTask& T, const PrimeRec& pr,
-                                                  uint32_t k, uint32_t span, uint32_t* cnt,
+__device__ __forceinline__ void bucket_fill_class(uint64_t w0, const PrimeRec& pr, uint32_t k,
+                                                  uint32_t span, uint32_t* cnt,
                                                   const uint32_t* lim, uint32_t* entries,
                                                   int log2w, uint32_t wmask) {
     const int j = (int)(k & 7u);
     const uint32_t p = pr.p;
-    const uint32_t r = mod_u64(T.w0, p, pr.magic);
+    const uint32_t r = mod_u64(w0, p, pr.magic);
     const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
-    uint32_t b = first_pos(p, r, j, T.w0);
+    uint32_t b = first_pos(p, r, j, w0);
     while ((uint64_t)b + 3ull * p < span) {
         uint32_t bb[4], sl[4], lm[4];
 #pragma unroll
@@ -711,20 +718,21 @@ __device__ __forceinline__ void bucket_fill_class(const BucketTask& T, const Pri
     }
 }
 
-// Every work item of a task, four per thread at a time: the four prime
-// records are loaded before any is used, so one load latency covers four
-// items (the items are short: the loads would otherwise dominate).
+// Every work item of the prime range [i0, i1) (8 per prime in class mode),
+// four per thread at a time: the four prime records are loaded before any
+// is used, so one load latency covers four items.
 template <typename G>
-__device__ __forceinline__ void bucket_sweep_items(const BucketTask& T,
+__device__ __forceinline__ void bucket_sweep_items(uint32_t i0, uint32_t i1, bool walk,
                                                    const PrimeRec* __restrict__ primes, G&& g) {
-    const uint32_t nthr = (T.i1 - T.i0) * (T.walk ? 1u : 8u);
-    const uint32_t last = T.i1 - 1u;
+    if (i1 <= i0) return;
+    const uint32_t nthr = (i1 - i0) * (walk ? 1u : 8u);
+    const uint32_t last = i1 - 1u;
     for (uint32_t k0 = threadIdx.x; k0 < nthr; k0 += 4u * blockDim.x) {
         PrimeRec pr[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const uint32_t k = k0 + q * blockDim.x;
-            const uint32_t i = T.i0 + (T.walk ? k : k >> 3);
+            const uint32_t i = i0 + (walk ? k : k >> 3);
             pr[q] = primes[i < last ? i : last];
         }
 #pragma unroll
@@ -735,67 +743,89 @@ __device__ __forceinline__ void bucket_sweep_items(const BucketTask& T,
     }
 }
 
-template <typename F>
-__device__ __forceinline__ void bucket_sweep(const BucketTask& T, const PrimeRec* __restrict__ primes,
-                                             uint32_t span, F&& f) {
-    bucket_sweep_items(T, primes,
-                       [&](const PrimeRec& pr, uint32_t k) { bucket_hits(T, pr, k, span, f); });
-}
-
 template <bool FILL>
 __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restrict__ tasks,
                                                       uint32_t ntask,
                                                       const PrimeRec* __restrict__ primes,
                                                       uint32_t* __restrict__ bucket_cnt,
                                                       const unsigned long long* __restrict__ bucket_off,
-                                                      uint32_t* __restrict__ bucket_cursor,
                                                       uint32_t* __restrict__ entries,
-                                                      uint32_t* __restrict__ overflow, int log2w) {
+                                                      uint32_t* __restrict__ overflow,
+                                                      uint32_t* __restrict__ task_next, int log2w) {
     __shared__ uint32_t cnt[kBucketTileMax];
     __shared__ uint32_t lim[FILL ? kBucketTileMax : 1];
+    __shared__ uint32_t s_task;
     const uint32_t tid = threadIdx.x;
     const uint32_t wmask = (1u << log2w) - 1u;
-    for (uint32_t t = blockIdx.x; t < ntask; t += gridDim.x) {
+    while (true) {
+        if (tid == 0) s_task = atomicAdd(task_next, 1u);
+        __syncthreads();
+        const uint32_t t = s_task;
+        if (t >= ntask) break;  // uniform: every thread read the same s_task
         const BucketTask T = tasks[t];
-        const uint32_t span = T.nwin << log2w;  // bytes; at most 2^30
-        for (uint32_t i = tid; i < T.nwin; i += blockDim.x) cnt[i] = 0;
-        __syncthreads();
-        bucket_sweep(T, primes, span,
-                     [&](uint32_t b, uint32_t) { atomicAdd(&cnt[b >> log2w], 1u); });
-        __syncthreads();
-        if (!FILL) {
-            for (uint32_t i = tid; i < T.nwin; i += blockDim.x)
-                if (cnt[i]) atomicAdd(&bucket_cnt[T.b0 + i], cnt[i]);
-            __syncthreads();
-            continue;
-        }
+        const uint32_t span = T.nwin << log2w;  // bytes; at most 2^25
+        auto count = [&](uint32_t b, uint32_t) { atomicAdd(&cnt[b >> log2w], 1u); };
         for (uint32_t i = tid; i < T.nwin; i += blockDim.x) {
-            if (!cnt[i]) continue;
-            const unsigned long long o0 = bucket_off[T.b0 + i], o1 = bucket_off[T.b0 + i + 1];
-            const uint32_t start = atomicAdd(&bucket_cursor[T.b0 + i], cnt[i]);
-            if ((unsigned long long)start + cnt[i] > o1 - o0) atomicOr(overflow, 1u);
-            cnt[i] = (uint32_t)o0 + start;
-            lim[i] = (uint32_t)o1;
+            if (FILL && !T.shared) {
+                cnt[i] = (uint32_t)bucket_off[T.b0 + i];
+                lim[i] = (uint32_t)bucket_off[T.b0 + i + 1];
+            } else {
+                cnt[i] = 0;
+            }
         }
         __syncthreads();
-        if (T.walk) {
-            bucket_sweep(T, primes, span, [&](uint32_t b, uint32_t bit) {
-                const uint32_t w = b >> log2w;
-                const uint32_t slot = atomicAdd(&cnt[w], 1u);
-                if (slot < lim[w]) entries[slot] = ((b & wmask) << 3) | bit;
+        if (FILL && T.shared) {
+            // count, then reserve this task's slots in each window's bucket
+            bucket_sweep_items(T.i0, T.iw, false, primes, [&](const PrimeRec& pr, uint32_t k) {
+                bucket_hits(false, T.w0, pr, k, span, count);
+            });
+            bucket_sweep_items(T.iw, T.i1, true, primes, [&](const PrimeRec& pr, uint32_t k) {
+                bucket_hits(true, T.w0, pr, k, span, count);
+            });
+            __syncthreads();
+            for (uint32_t i = tid; i < T.nwin; i += blockDim.x) {
+                const unsigned long long o0 = bucket_off[T.b0 + i], o1 = bucket_off[T.b0 + i + 1];
+                const uint32_t start = cnt[i] ? atomicAdd(&bucket_cnt[T.b0 + i], cnt[i]) : 0u;
+                if ((unsigned long long)start + cnt[i] > o1 - o0) atomicOr(overflow, 1u);
+                cnt[i] = (uint32_t)o0 + start;
+                lim[i] = (uint32_t)o1;
+            }
+            __syncthreads();
+        }
+        if (!FILL) {
+            bucket_sweep_items(T.i0, T.iw, false, primes, [&](const PrimeRec& pr, uint32_t k) {
+                bucket_hits(false, T.w0, pr, k, span, count);
+            });
+            bucket_sweep_items(T.iw, T.i1, true, primes, [&](const PrimeRec& pr, uint32_t k) {
+                bucket_hits(true, T.w0, pr, k, span, count);
             });
         } else {
-            bucket_sweep_items(T, primes, [&](const PrimeRec& pr, uint32_t k) {
-                bucket_fill_class(T, pr, k, span, cnt, lim, entries, log2w, wmask);
+            bucket_sweep_items(T.i0, T.iw, false, primes, [&](const PrimeRec& pr, uint32_t k) {
+                bucket_fill_class(T.w0, pr, k, span, cnt, lim, entries, log2w, wmask);
+            });
+            bucket_sweep_items(T.iw, T.i1, true, primes, [&](const PrimeRec& pr, uint32_t k) {
+                bucket_hits(true, T.w0, pr, k, span, [&](uint32_t b, uint32_t bit) {
+                    const uint32_t w = b >> log2w;
+                    const uint32_t slot = atomicAdd(&cnt[w], 1u);
+                    if (slot < lim[w]) entries[slot] = ((b & wmask) << 3) | bit;
+                });
             });
         }
         __syncthreads();
+        for (uint32_t i = tid; i < T.nwin; i += blockDim.x) {
+            if (T.shared) {
+                if (!FILL && cnt[i]) atomicAdd(&bucket_cnt[T.b0 + i], cnt[i]);
+            } else if (FILL) {
+                bucket_cnt[T.b0 + i] = cnt[i] - (uint32_t)bucket_off[T.b0 + i];
+                if (cnt[i] > lim[i]) atomicOr(overflow, 1u);
+            } else {
+                bucket_cnt[T.b0 + i] = cnt[i];
+            }
+        }
+        __syncthreads();  // cnt and s_task are reused by the next task
     }
 }
 
-// dst[j] += src[j] for the accumulators of n jobs: sums, min of first, max
-// of last, or of the error bits.  Used to fold a step's sums into
-// long-lived per-request accumulators.
 __global__ void add_acc_kernel(JobAcc* __restrict__ dst, const JobAcc* __restrict__ src) {
     JobAcc& d = dst[blockIdx.x];
     const JobAcc& s = src[blockIdx.x];
@@ -907,22 +937,23 @@ hipError_t launch_stitch(const SieveArgs& a, hipStream_t s) {
 namespace voss {
 
 hipError_t launch_bucket_count(const BucketTask* tasks, uint32_t ntask, const PrimeRec* primes,
-                               uint32_t* bucket_cnt, int log2w, uint32_t grid, hipStream_t s) {
+                               uint32_t* bucket_cnt, uint32_t* task_next, int log2w, uint32_t grid,
+                               hipStream_t s) {
     if (ntask == 0) return hipSuccess;
     if (grid > ntask) grid = ntask;
     hipLaunchKernelGGL(bucket_kernel<false>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes,
-                       bucket_cnt, nullptr, nullptr, nullptr, nullptr, log2w);
+                       bucket_cnt, nullptr, nullptr, nullptr, task_next, log2w);
     return hipGetLastError();
 }
 
 hipError_t launch_bucket_fill(const BucketTask* tasks, uint32_t ntask, const PrimeRec* primes,
                               const unsigned long long* bucket_off, uint32_t* bucket_cursor,
-                              uint32_t* entries, uint32_t* overflow, int log2w, uint32_t grid,
-                              hipStream_t s) {
+                              uint32_t* entries, uint32_t* overflow, uint32_t* task_next, int log2w,
+                              uint32_t grid, hipStream_t s) {
     if (ntask == 0) return hipSuccess;
     if (grid > ntask) grid = ntask;
     hipLaunchKernelGGL(bucket_kernel<true>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes,
-                       nullptr, bucket_off, bucket_cursor, entries, overflow, log2w);
+                       bucket_cursor, bucket_off, entries, overflow, task_next, log2w);
     return hipGetLastError();
 }
 

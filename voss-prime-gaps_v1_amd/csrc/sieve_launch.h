@@ -43,18 +43,21 @@ hipError_t launch_stitch(const SieveArgs& a, hipStream_t s);
 hipError_t launch_sieve_emit(const SieveArgs& a, int window_bytes, uint32_t grid,
                              hipStream_t s);
 
-// Bucket passes over `ntask` tasks.  count: bucket_cnt[b] += hits.  fill:
-// writes entries using bucket_off (nbuckets + 1 offsets) and the per-bucket
-// cursors; a bucket that would exceed its room sets *overflow instead.
+// Bucket passes over `ntask` tasks (one workgroup owns a task's tile; tasks
+// are taken from *task_next, which must be 0 at launch).  count:
+// bucket_cnt[b] = hits of window b.  fill: writes entries using bucket_off
+// (nbuckets + 1 offsets) and sets bucket_cursor[b] to the hits of window b;
+// a bucket that would exceed its room sets *overflow instead.
 hipError_t launch_bucket_count(const BucketTask* tasks, uint32_t ntask,
                                const PrimeRec* primes, uint32_t* bucket_cnt,
-                               int log2w, uint32_t grid, hipStream_t s);
+                               uint32_t* task_next, int log2w, uint32_t grid,
+                               hipStream_t s);
 hipError_t launch_bucket_fill(const BucketTask* tasks, uint32_t ntask,
                               const PrimeRec* primes,
                               const unsigned long long* bucket_off,
                               uint32_t* bucket_cursor, uint32_t* entries,
-                              uint32_t* overflow, int log2w, uint32_t grid,
-                              hipStream_t s);
+                              uint32_t* overflow, uint32_t* task_next, int log2w,
+                              uint32_t grid, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 // dst[j] += src[j] for j < njobs (sums; min first, max last, or error).
 hipError_t launch_add_acc(JobAcc* dst, const JobAcc* src, uint32_t njobs, hipStream_t s);

@@ -62,16 +62,22 @@ constexpr uint32_t kNoBucket = 0xFFFFFFFFu;
 // Large-prime bucket fill: one task = the primes [i0, i1) of one job over
 // `nwin` consecutive grid windows starting at byte w0 (buckets b0..).
 // A bucket entry is (byte offset in the window << 3) | wheel bit.
+// One bucket task: a tile of nwin consecutive windows of one job (buckets
+// b0 .. b0+nwin-1) and large primes [i0, iw) in class mode, [iw, i1) in
+// walk mode.  An owner task holds all the tile's primes, so its buckets see
+// no other writer; a tile with too much work for one workgroup is split
+// into shared tasks (prime chunks) that reserve their slots with global
+// atomics on the buckets' cursors.
 struct BucketTask {
     uint64_t w0;
     uint32_t b0;
     uint32_t nwin;
     uint32_t i0;
+    uint32_t iw;
     uint32_t i1;
-    uint32_t walk;  // 1: thread per prime stepping over all its multiples
-    uint32_t pad;
+    uint32_t shared;  // 1: one of several tasks of its tile
 };
-constexpr int kBucketTileMax = 8192;  // windows per task (LDS counters)
+constexpr int kBucketTileMax = 256;  // windows per task (LDS counters)
 
 // Per base prime, one 16-byte record read with a single load: the prime,
 // the wheel bit of p*W30[j] for each residue class j (3 bits each) and the
