@@ -211,6 +211,13 @@ int voss_batch_prepare(voss_ctx* ctx, const uint64_t* lo, const uint64_t* hi,
                        size_t n, voss_batch** out);
 int voss_batch_run(voss_batch* batch, voss_interval_result* out,
                    voss_result* stats);
+/* Adds a Miller-Rabin batch over device-resident candidates (voss_device_alloc
+ * buffers) to a prepared batch: every later voss_batch_run also sets
+ * d_verdict[i] = 1 iff d_n[i] is prime (is_prime_u64, src/verify_cpp.cpp:38-57),
+ * on a second stream beside the sieve rounds, inside the same captured
+ * graph.  The buffers must stay valid until voss_batch_free. */
+int voss_batch_add_mr(voss_batch* batch, const uint64_t* d_n, size_t count,
+                      uint8_t* d_verdict);
 void voss_batch_free(voss_batch* batch);
 
 /* ----------------------------------------------------------- Miller-Rabin */

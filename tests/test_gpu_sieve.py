@@ -372,3 +372,54 @@ def test_bucket_overflow_reruns_exactly(engine, oracle, graph):
     longest = sorted(range(len(lo)), key=lambda k: hi[k] - lo[k])[-2:]
     for k in longest:
         assert_same(got[k], oracle.run_interval(lo[k], hi[k]))
+
+
+def test_prepared_batch_with_mr(engine):
+    """A prepared batch with two Miller-Rabin batches on device buffers:
+    every replay gives the interval results of the eager batch and the
+    verdicts of voss_mr_batch; a batch with MR work only also runs."""
+    import ctypes
+    import vossgpu
+    L = vossgpu.lib()
+    lo, hi = _mixed_batch(0x564F5353 + 27)
+    ref, _ = engine.run_batch(lo, hi, large_cap=256)
+    rng = np.random.default_rng(0x564F5353 + 28)
+    c1 = (rng.integers(1 << 40, 1 << 63, size=20000, dtype=np.uint64) * 2 + 1).astype(np.uint64)
+    c2 = np.array([2, 3, 4, 37, 41, 3825123056546413051, 2**61 - 1, 2**64 - 59,
+                   (2**32 - 5) * (2**32 - 17)], dtype=np.uint64)
+    want = [engine.mr(c1), engine.mr(c2)]
+    bufs = []
+    try:
+        for c in (c1, c2):
+            dn, dv = ctypes.c_void_p(), ctypes.c_void_p()
+            vossgpu._check(L.voss_device_alloc(engine._h, c.nbytes, ctypes.byref(dn)))
+            vossgpu._check(L.voss_device_alloc(engine._h, max(c.size, 8), ctypes.byref(dv)))
+            vossgpu._check(L.voss_memcpy_h2d(engine._h, dn, c.ctypes.data, c.nbytes))
+            bufs.append((dn, dv, c.size))
+        with engine.prepare_batch(lo, hi) as pb:
+            for dn, dv, n in bufs:
+                pb.add_mr(dn, n, dv)
+            for rep in range(3):
+                print(f"[gpu] prepared batch + MR replay {rep}", flush=True)
+                for dn, dv, n in bufs:  # stale verdicts must be overwritten
+                    z = np.full(n, 7, dtype=np.uint8)
+                    vossgpu._check(L.voss_memcpy_h2d(engine._h, dv, z.ctypes.data, n))
+                got, _ = pb.run(large_cap=256)
+                for rg, rr in zip(got, ref):
+                    assert_same(rg, rr)
+                for (dn, dv, n), w in zip(bufs, want):
+                    v = np.zeros(n, dtype=np.uint8)
+                    vossgpu._check(L.voss_memcpy_d2h(engine._h, v.ctypes.data, dv, n))
+                    assert np.array_equal(v, w)
+                engine.mr(c2[:3])  # eager work between replays
+        with engine.prepare_batch([], []) as pb:
+            dn, dv, n = bufs[1]
+            pb.add_mr(dn, n, dv)
+            pb.run()
+            v = np.zeros(n, dtype=np.uint8)
+            vossgpu._check(L.voss_memcpy_d2h(engine._h, v.ctypes.data, dv, n))
+            assert np.array_equal(v, want[1])
+    finally:
+        for dn, dv, _ in bufs:
+            L.voss_device_free(engine._h, dn)
+            L.voss_device_free(engine._h, dv)

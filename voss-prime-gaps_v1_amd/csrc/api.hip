@@ -734,9 +734,52 @@ struct Executor {
     const PrimeRec* graph_primes = nullptr;  // base-prime table the graph reads
     bool copy_acc = true;      // read the accumulators back after a run
     uint64_t nlarge = 0;       // large-gap records of the last run (in B->h_large)
+    // Miller-Rabin batches run beside the rounds on a second stream (forked
+    // from and joined back into the main one, so a capture holds both)
+    struct MrJob {
+        const uint64_t* d_n;
+        uint64_t count;
+        uint8_t* d_verdict;
+        uint32_t* d_surv;
+        uint32_t* d_nsurv;
+    };
+    std::vector<MrJob> mr;
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 
     Executor(voss_ctx* ctx, ExecBufs* bufs) : c(ctx), B(bufs) {}
-    ~Executor() { drop_graph(); }
+    ~Executor() {
+        drop_graph();
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        if (join_ev) (void)hipEventDestroy(join_ev);
+        if (side) (void)hipStreamDestroy(side);
+    }
+
+    int ensure_side() {
+        if (side) return VOSS_OK;
+        HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+        return VOSS_OK;
+    }
+
+    // fork: the side stream waits for the main one, runs the MR batches
+    int enqueue_mr_fork() {
+        if (mr.empty()) return VOSS_OK;
+        HIPCHK(hipEventRecord(fork_ev, c->stream));
+        HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
+        for (const MrJob& m : mr)
+            HIPCHK(launch_mr(m.d_n, m.count, m.d_verdict, m.d_surv, m.d_nsurv, c->num_cus, side));
+        return VOSS_OK;
+    }
+
+    // join: the main stream waits for the side stream
+    int enqueue_mr_join() {
+        if (mr.empty()) return VOSS_OK;
+        HIPCHK(hipEventRecord(join_ev, side));
+        HIPCHK(hipStreamWaitEvent(c->stream, join_ev, 0));
+        return VOSS_OK;
+    }
     Executor(const Executor&) = delete;
     Executor& operator=(const Executor&) = delete;
 
@@ -842,8 +885,9 @@ struct Executor {
         const int log2w = log2_window(c);
         const uint32_t grid = (uint32_t)c->num_cus * std::max<uint32_t>(c->tune.wg_per_cu, 1);
         const uint32_t bgrid = (uint32_t)c->num_cus * 2;
-        HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, nd * sizeof(SegDesc),
-                              hipMemcpyHostToDevice, s));
+        if (nd)
+            HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, nd * sizeof(SegDesc),
+                                  hipMemcpyHostToDevice, s));
         if (ntask) {
             HIPCHK(hipMemcpyAsync(B->d_btask.p, B->h_btask.p, ntask * sizeof(BucketTask),
                                   hipMemcpyHostToDevice, s));
@@ -852,6 +896,8 @@ struct Executor {
         }
         HIPCHK(launch_fill_u32((uint32_t*)B->d_flags.p, 4, 0, s));
         HIPCHK(launch_init_acc(B->d_acc.p, P.njobs, s));
+        int rc = enqueue_mr_fork();
+        if (rc) return rc;
         for (size_t r = 0; r < rounds.size(); r++) {
             const Round& R = rounds[r];
             if (timed) HIPCHK(hipEventRecord(ev(4 * r), s));
@@ -870,16 +916,18 @@ struct Executor {
             HIPCHK(launch_stitch(a, s));
             if (timed) HIPCHK(hipEventRecord(ev(4 * r + 3), s));
         }
+        if ((rc = enqueue_mr_join())) return rc;
         return enqueue_readback();
     }
 
     int enqueue_readback() {
         hipStream_t s = c->stream;
-        if (copy_acc)
+        if (copy_acc && P.njobs)
             HIPCHK(hipMemcpyAsync(B->h_acc.p, B->d_acc.p, P.njobs * sizeof(JobAcc),
                                   hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(B->h_rec.p, B->d_rec.p, P.desc.size() * sizeof(SegRec),
-                              hipMemcpyDeviceToHost, s));
+        if (!P.desc.empty())
+            HIPCHK(hipMemcpyAsync(B->h_rec.p, B->d_rec.p, P.desc.size() * sizeof(SegRec),
+                                  hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(B->h_flags.p, B->d_flags.p, 16, hipMemcpyDeviceToHost, s));
         return VOSS_OK;
     }
@@ -1635,6 +1683,10 @@ struct voss_batch {
     uint64_t nwin = 0;
     ExecBufs B;
     std::unique_ptr<Executor> X;
+    std::vector<std::unique_ptr<DevBuf<uint32_t>>> mr_scratch;  // survivors + counter per MR batch
+    ~voss_batch() {
+        for (auto& m : mr_scratch) m->release();
+    }
 };
 
 extern "C" {
@@ -1670,7 +1722,7 @@ int voss_batch_run(voss_batch* b, voss_interval_result* out, voss_result* stats)
     const double t0 = now_s();
     std::vector<JobHost> jobs;
     ExecTimes tm;
-    if (b->X->P.desc.empty()) {
+    if (b->X->P.desc.empty() && b->X->mr.empty()) {
         jobs.assign(n, JobHost());
     } else {
         if ((rc = b->X->run(true, &tm))) return rc;
@@ -1678,6 +1730,21 @@ int voss_batch_run(voss_batch* b, voss_interval_result* out, voss_result* stats)
     }
     if ((rc = finish_batch(jobs, b->lo.data(), b->hi.data(), n, out))) return rc;
     batch_stats(stats, tm, t0, b->nwin);
+    return VOSS_OK;
+}
+
+int voss_batch_add_mr(voss_batch* b, const uint64_t* d_n, size_t count, uint8_t* d_verdict) {
+    if (!b) return fail(VOSS_EINVAL, "batch is NULL");
+    if (count && (!d_n || !d_verdict)) return fail(VOSS_EINVAL, "NULL buffer");
+    if (count >= 0xFFFFFFFFull) return fail(VOSS_EINVAL, "count must be below 2^32");
+    if (count == 0) return VOSS_OK;
+    int rc = set_device(b->c);
+    if (rc || (rc = b->X->ensure_side())) return rc;
+    std::unique_ptr<DevBuf<uint32_t>> scratch(new DevBuf<uint32_t>());
+    HIPCHK(scratch->reserve(count + 1));  // survivors, then the survivor counter
+    b->X->mr.push_back({d_n, count, d_verdict, scratch->p, scratch->p + count});
+    b->mr_scratch.push_back(std::move(scratch));
+    b->X->drop_graph();  // the next run captures the sequence with this batch in it
     return VOSS_OK;
 }
 

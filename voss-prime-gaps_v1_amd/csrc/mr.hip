@@ -16,6 +16,7 @@
 
 #include "montgomery.h"
 #include "mr_launch.h"
+#include "sieve_launch.h"
 
 namespace voss {
 
@@ -166,7 +167,8 @@ hipError_t launch_mr(const uint64_t* d_cand, uint64_t count, uint8_t* d_verdict,
                      uint32_t* d_surv, uint32_t* d_nsurv, int num_cus,
                      hipStream_t s) {
     if (count == 0) return hipSuccess;
-    hipError_t e = hipMemsetAsync(d_nsurv, 0, sizeof(uint32_t), s);
+    // a fill kernel, not hipMemsetAsync, so the sequence can be captured
+    hipError_t e = launch_fill_u32(d_nsurv, 1, 0, s);
     if (e != hipSuccess) return e;
     uint64_t blocks = (count + 255) / 256;
     const uint64_t cap = (uint64_t)num_cus * 32;

@@ -137,6 +137,7 @@ EXPORTS = [
     "voss_memcpy_h2d", "voss_memcpy_d2h", "voss_serve_start", "voss_serve_submit_interval",
     "voss_serve_submit_mr", "voss_serve_poll", "voss_serve_wait", "voss_serve_stop",
     "voss_set_graph_mode", "voss_batch_prepare", "voss_batch_run", "voss_batch_free",
+    "voss_batch_add_mr",
 ]
 
 _lib = None
@@ -170,6 +171,7 @@ def lib():
         L.voss_batch_prepare.argtypes = [vp, vp, vp, sz, P(vp)]
         L.voss_batch_run.argtypes = [vp, vp, P(Result)]
         L.voss_batch_free.argtypes = [vp]
+        L.voss_batch_add_mr.argtypes = [vp, vp, sz, vp]
         L.voss_batch_free.restype = None
         L.voss_mr_batch.argtypes = [vp, vp, sz, vp]
         L.voss_mr_batch_device.argtypes = [vp, vp, sz, vp, P(ctypes.c_double)]
@@ -434,6 +436,10 @@ class PreparedBatch:
         if raw:
             return res, lbuf, st
         return [interval_to_dict(res[k]) for k in range(n)], st
+
+    def add_mr(self, d_n, count, d_verdict):
+        """Miller-Rabin over device buffers (voss_device_alloc) in every run."""
+        _check(lib().voss_batch_add_mr(self._b, d_n, count, d_verdict))
 
     def close(self):
         if self._b:
