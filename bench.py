@@ -140,7 +140,7 @@ def cpu_baseline(threads):
                       f"{threads} threads"}
 
 
-def batch_leg(eng, W, with_plain=True, autotune=True):
+def batch_leg(eng, W, with_plain=True, autotune=True, mr_count=10**7):
     """Config 4: 4096 seeded intervals (starts in [1e12, 1e16), lengths
     log-uniform in [1e4, 1e9]).  The autotuner picks window, workgroup and
     bucket thresholds on a 256-interval sample; then the batch runs as one
@@ -185,6 +185,10 @@ def batch_leg(eng, W, with_plain=True, autotune=True):
                 reps.append(time.perf_counter() - t)
             out.update({"replay_s": min(reps), "replay_numbers_per_s": numbers / min(reps),
                         "replay_bit_exact": bool(same(rb, rr))})
+        # the same batch with config 4's Miller-Rabin set in the same graph
+        # (MR on a second stream beside the sieve rounds)
+        cand = W.mr_candidates(mr_count, large_endpoints(rb))
+        out["with_mr"] = batch_with_mr(eng, lo, hi, cand, rb, same, min(reps))
         eng.set_graph_mode(False)
         try:
             t = time.perf_counter()
@@ -201,13 +205,49 @@ def batch_leg(eng, W, with_plain=True, autotune=True):
                         "bit_exact": bool(same(rb, rp))})
     finally:
         eng.set_tuning(**default)
-    # gap endpoints for the Miller-Rabin leg
+    return out, large_endpoints(rb)
+
+
+def large_endpoints(res):
+    """Both ends of every large gap kept in a batch result (MR candidates)."""
     ends = []
-    for k in range(len(iv)):
-        for i in range(min(rb[k].large_count, rb[k].large_capacity)):
-            g = rb[k].large[i]
+    for r in res:
+        for i in range(min(r.large_count, r.large_capacity)):
+            g = r.large[i]
             ends += [g.position, g.position - g.gap]
-    return out, np.array(ends, dtype=np.uint64)
+    return np.array(ends, dtype=np.uint64)
+
+
+def batch_with_mr(eng, lo, hi, cand, rb, same, replay_s):
+    """Prepared batch + device-resident MR batch in one graph: replay time,
+    against the batch alone plus MR alone; verdicts against voss_mr_batch."""
+    import ctypes
+    import vossgpu
+    L = vossgpu.lib()
+    n = len(cand)
+    dn, dv = ctypes.c_void_p(), ctypes.c_void_p()
+    vossgpu._check(L.voss_device_alloc(eng._h, n * 8, ctypes.byref(dn)))
+    vossgpu._check(L.voss_device_alloc(eng._h, n, ctypes.byref(dv)))
+    try:
+        vossgpu._check(L.voss_memcpy_h2d(eng._h, dn, cand.ctypes.data, n * 8))
+        _, mr_ms = eng.mr_device(cand[: min(n, 10**6)])  # warm-up
+        verdict, mr_ms = eng.mr_device(cand, repeats=3)
+        with eng.prepare_batch(lo, hi) as pb:
+            pb.add_mr(dn, n, dv)
+            pb.run(large_cap=64, raw=True)  # capture
+            reps = []
+            for _ in range(3):
+                t = time.perf_counter()
+                rr, lbuf, _ = pb.run(large_cap=64, raw=True)
+                reps.append(time.perf_counter() - t)
+        v = np.zeros(n, dtype=np.uint8)
+        vossgpu._check(L.voss_memcpy_d2h(eng._h, v.ctypes.data, dv, n))
+    finally:
+        L.voss_device_free(eng._h, dn)
+        L.voss_device_free(eng._h, dv)
+    separate = replay_s + statistics.median(mr_ms) / 1e3
+    return {"mr_count": n, "replay_s": min(reps), "batch_plus_mr_separately_s": separate,
+            "bit_exact": bool(same(rb, rr)), "verdicts_equal": bool(np.array_equal(v, verdict))}
 
 
 def serve_leg(eng, W, n_req=300, n_cal=80):
@@ -348,7 +388,8 @@ def main():
         legs["interval_1e13"] = interval_leg(eng)
     if "batch" in want:
         legs["batch_4096"], ends = batch_leg(eng, W, with_plain=not args.no_plain,
-                                             autotune=not args.no_autotune)
+                                             autotune=not args.no_autotune,
+                                             mr_count=args.mr_count)
     if "serve" in want:
         legs["serve"] = serve_leg(eng, W)
 
