@@ -813,7 +813,17 @@ struct Executor {
     int stage() {
         const size_t nd = P.desc.size();
         const uint32_t nj = std::max<uint32_t>(P.njobs, 1);
-        if (lcap == 0) lcap = std::max<uint64_t>(B->d_large.cap, 1u << 20);
+        if (lcap == 0) {
+            // room for every large-gap candidate: about 1.5e-5 of the primes
+            // at 1e11..1e14 (8.9e-6 at 1e11, 1.3e-5 at 1e13); 2e-5 of a
+            // prime-count estimate avoids a re-run for overflow in practice
+            double primes = 0;  // every 16th window, scaled
+            for (size_t i = 0; i < P.desc.size(); i += 16) {
+                const SegDesc& d = P.desc[i];
+                primes += 16.0 * (double)(d.n_hi - d.n_lo + 1) / std::log((double)d.n_hi + 3.0);
+            }
+            lcap = std::max<uint64_t>({B->d_large.cap, 1u << 20, (uint64_t)(2e-5 * primes)});
+        }
         uint64_t ent = 1;
         uint32_t nb_max = 1;
         for (const Round& R : rounds) {
