@@ -33,6 +33,7 @@ sys.path.insert(0, PKG)
 import numpy as np  # noqa: E402
 
 N_BENCH = 10**11
+REF_NUMBERS_PER_S = 10**11 / 5.8  # README.md:124, A100-SXM4-40GB
 SEG_NUM = 30000000000
 CPU_SAMPLE_N = 10**10
 
@@ -351,7 +352,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="run the batch leg with the default tuning")
     ap.add_argument("--legs", default="interval,batch,serve",
-                    help="comma-separated legs after the headline: interval, batch, serve")
+                    help="comma-separated legs after the headline: interval, batch, serve, "
+                         "plain1e14 (not default: about 35 s)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -392,6 +394,14 @@ def main():
                                              mr_count=args.mr_count)
     if "serve" in want:
         legs["serve"] = serve_leg(eng, W)
+    if "plain1e14" in want:  # not in the default set: ~35 s
+        t = time.perf_counter()
+        r14 = eng.run_plain(10**14, SEG_NUM, large_cap=16)
+        dt = time.perf_counter() - t
+        assert r14["total_primes"] == 3204941750802, r14["total_primes"]
+        legs["plain_1e14"] = {"seconds": dt, "numbers_per_s": 1e14 / dt,
+                              "reference_seconds": 21671.0,  # README.md:127, A100-80GB
+                              "windows": r14["windows"], "large_gaps": r14["large_count"]}
 
     # Miller-Rabin tests/s on device-resident candidates (config 4's set)
     cand = W.mr_candidates(args.mr_count, ends)
@@ -438,7 +448,9 @@ def main():
         "ms_per_step": 1e3 * total / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        # the reference's published pi(1e11) wall time: 5.8 s on one A100
+        # (README.md:124) -> 1.72e10 numbers/s for the same [1, 1e11] pipeline
+        "vs_baseline": (N_BENCH * args.steps / total) / REF_NUMBERS_PER_S,
         "dtype": "u32/u64 integer",
         "data": "synthetic: the range [1, 1e11] itself; batch intervals and MR candidates "
                 "from seeded SplitMix64 streams (seed 0x564F5353+k)",
