@@ -170,7 +170,7 @@ bool merit_ge10(uint64_t p_before, uint32_t g) {
 // large enough to pay for starting them.
 template <typename F>
 void parallel_for(size_t n, F&& f) {
-    const size_t kMinPerThread = 32768;
+    const size_t kMinPerThread = 8192;
     size_t nt = std::min<size_t>(16, std::max<unsigned>(1u, std::thread::hardware_concurrency()));
     nt = std::min(nt, std::max<size_t>(1, n / kMinPerThread));
     if (nt <= 1) {
@@ -210,6 +210,9 @@ struct voss_ctx {
     DevBuf<unsigned long long> d_lg_thresh;
 
     ExecBufs eb;                  // buffers of the context's own runs
+    // window list of the last plain run, kept so the next plan reuses its
+    // memory (a fresh 1 MB vector per call costs page faults)
+    std::vector<SegDesc> desc_pool;
     std::vector<hipEvent_t> evs;  // per-round timing events
     int graph_mode = 1;           // batched runs as one captured hipGraph
 
@@ -255,7 +258,7 @@ int ensure_primes(voss_ctx* c, uint64_t max_number) {
         for (int j = 0; j < 8; j++)
             w |= (uint32_t)residx[(uint64_t)p * (uint64_t)kW30[j] % 30] << (3 * j);
         rec[i].p = p;
-        rec[i].perm = w;
+        rec[i].perm = w | (p % 30u) << 24;
         rec[i].magic = ~0ull / p;
     }
     const size_t n = std::max<size_t>(c->primes.size(), 1);
@@ -400,34 +403,51 @@ void add_acc(JobHost& J, const JobAcc& A) {
 // ascending (the reference sorts by merit alone with an unstable sort,
 // src/voss_v7_live.cu:813, so ties have no defined order there).  Merits
 // are positive floats, whose bit patterns order like the values: an LSD
-// radix sort of the 64-bit keys (inverted merit bits : list index), byte
-// passes whose digit never varies skipped, then runs of equal merit sorted
-// by position.
+// radix sort (11-bit digits, digits that never vary skipped) of the
+// inverted merit bits carrying the list index, then runs of equal merit
+// sorted by position.
 void sort_large(std::vector<voss_large_gap>& v) {
     const size_t n = v.size();
     if (n < 2) return;
-    std::vector<uint64_t> k(n), t(n);
-    uint64_t all_or = 0, all_and = ~0ull;
+    // scratch kept per thread for lists up to 2^20 records (fresh multi-MB
+    // vectors per call cost page faults), local beyond
+    static thread_local std::vector<uint32_t> keep_u32[4];
+    static thread_local std::vector<voss_large_gap> keep_out;
+    std::vector<uint32_t> own_u32[4];
+    std::vector<voss_large_gap> own_out;
+    const bool keep = n <= (1u << 20);
+    std::vector<uint32_t>* u = keep ? keep_u32 : own_u32;
+    std::vector<voss_large_gap>& out = keep ? keep_out : own_out;
+    for (int q = 0; q < 4; q++) u[q].resize(n);
+    uint32_t *key = u[0].data(), *idx = u[1].data(), *key2 = u[2].data(), *idx2 = u[3].data();
+    uint32_t all_or = 0, all_and = ~0u;
     for (size_t i = 0; i < n; i++) {
         uint32_t mb;
         std::memcpy(&mb, &v[i].merit, 4);
-        k[i] = ((uint64_t)(0xFFFFFFFFu - mb) << 32) | (uint64_t)i;
-        all_or |= k[i];
-        all_and &= k[i];
+        key[i] = 0xFFFFFFFFu - mb;
+        idx[i] = (uint32_t)i;
+        all_or |= key[i];
+        all_and &= key[i];
     }
-    for (int sh = 0; sh < 64; sh += 8) {
-        if ((((all_or ^ all_and) >> sh) & 0xFF) == 0) continue;  // digit constant
-        size_t cnt[257] = {};
-        for (size_t i = 0; i < n; i++) cnt[((k[i] >> sh) & 0xFF) + 1]++;
-        for (int d = 0; d < 256; d++) cnt[d + 1] += cnt[d];
-        for (size_t i = 0; i < n; i++) t[cnt[(k[i] >> sh) & 0xFF]++] = k[i];
-        k.swap(t);
+    for (int sh = 0; sh < 32; sh += 11) {
+        const uint32_t dmask = 0x7FFu;
+        if ((((all_or ^ all_and) >> sh) & dmask) == 0) continue;  // digit constant
+        size_t cnt[2049] = {};
+        for (size_t i = 0; i < n; i++) cnt[((key[i] >> sh) & dmask) + 1]++;
+        for (int d = 0; d < 2048; d++) cnt[d + 1] += cnt[d];
+        for (size_t i = 0; i < n; i++) {
+            const size_t at = cnt[(key[i] >> sh) & dmask]++;
+            key2[at] = key[i];
+            idx2[at] = idx[i];
+        }
+        std::swap(key, key2);
+        std::swap(idx, idx2);
     }
-    std::vector<voss_large_gap> out(n);
-    for (size_t i = 0; i < n; i++) out[i] = v[(uint32_t)k[i]];
+    out.resize(n);
+    for (size_t i = 0; i < n; i++) out[i] = v[idx[i]];
     for (size_t i = 0; i < n;) {
         size_t j = i + 1;
-        while (j < n && (k[j] >> 32) == (k[i] >> 32)) j++;
+        while (j < n && key[j] == key[i]) j++;
         if (j - i > 1)
             std::sort(out.begin() + i, out.begin() + j,
                       [](const voss_large_gap& x, const voss_large_gap& y) {
@@ -577,12 +597,12 @@ struct Round {
     uint64_t room_total = 0;
 };
 
-// Estimated VALU work of a bucket task: per-item setup (8 items per prime
-// in class mode, 1 in walk mode), class-mode hits, and walk steps (30
+// Estimated VALU work of a bucket task: per-prime setup (a class-mode
+// prime also starts eight classes), class-mode hits, and walk steps (30
 // span/p per walked prime).
 double bucket_task_cost(const voss_ctx* c, const BucketTask& T, uint32_t W) {
     const double span = (double)T.nwin * W;
-    return 25.0 * (8.0 * (double)(T.iw - T.i0) + (double)(T.i1 - T.iw)) +
+    return 25.0 * (4.0 * (double)(T.iw - T.i0) + (double)(T.i1 - T.iw)) +
            40.0 * span * (c->inv_prefix[T.iw] - c->inv_prefix[T.i0]) +
            210.0 * span * (c->inv_prefix[T.i1] - c->inv_prefix[T.iw]);
 }
@@ -604,9 +624,11 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
         R.off = off;
         uint64_t est = 0;
         size_t end = off;
+        bool any_bucket = false;
         while (end < P.desc.size() && end - off < kMaxDescPerLaunch) {
             const SegDesc& d = P.desc[end];
             if (bucketed(d)) {
+                any_bucket = true;
                 const uint64_t e = bucket_room(c->inv_prefix[d.nprimes] - c->inv_prefix[ibl], W,
                                                c->tune.bucket_room_pct);
                 if (end > off && est + e > kBucketEntryBudget * 15 / 16) break;
@@ -615,6 +637,11 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
             end++;
         }
         R.nd = end - off;
+        if (!any_bucket) {  // the plan's windows as they are (R.desc stays empty)
+            rounds.push_back(std::move(R));
+            off = end;
+            continue;
+        }
         R.desc.assign(P.desc.begin() + off, P.desc.begin() + end);
         std::vector<std::pair<double, BucketTask>> work;  // tasks with their estimated cost
         // buckets: consecutive windows of one job with the same w0 share one
@@ -850,7 +877,8 @@ struct Executor {
         HIPCHK(B->h_acc.reserve(nj));
         HIPCHK(B->h_flags.reserve(2));
         for (const Round& R : rounds) {
-            std::memcpy(B->h_desc.p + R.off, R.desc.data(), R.nd * sizeof(SegDesc));
+            std::memcpy(B->h_desc.p + R.off, R.desc.empty() ? P.desc.data() + R.off : R.desc.data(),
+                        R.nd * sizeof(SegDesc));
             if (!R.tasks.empty())
                 std::memcpy(B->h_btask.p + R.task_off, R.tasks.data(),
                             R.tasks.size() * sizeof(BucketTask));
@@ -1095,15 +1123,33 @@ struct Executor {
             if (B->h_acc.p[j].error) return fail(VOSS_ERANGE, "prime gap >= 2048 (beyond the device histogram)");
             add_acc(jobs[j], B->h_acc.p[j]);
         }
-        // merits (one log each) on several host threads for long lists
-        std::vector<float> merit(nlarge);
-        parallel_for(nlarge, [&](size_t k0, size_t k1) {
-            for (size_t k = k0; k < k1; k++)
-                merit[k] = merit_of(B->h_large.p[k].p_before, B->h_large.p[k].gap);
-        });
-        for (uint64_t k = 0; k < nlarge; k++) {
-            const LargeRaw& r = B->h_large.p[k];
-            jobs[r.job].large.push_back({r.p_before + r.gap, (int32_t)r.gap, merit[k]});
+        // large records with their merits (one log each, on several host
+        // threads for long lists), straight into the job lists: a single
+        // job's list in device order, several jobs' via per-job offsets
+        const LargeRaw* raw = B->h_large.p;
+        auto fill = [&](voss_large_gap* dst, const uint32_t* at) {
+            parallel_for(nlarge, [&](size_t k0, size_t k1) {
+                for (size_t k = k0; k < k1; k++) {
+                    const LargeRaw& r = raw[k];
+                    dst[at ? at[k] : k] = {r.p_before + r.gap, (int32_t)r.gap,
+                                           merit_of(r.p_before, r.gap)};
+                }
+            });
+        };
+        if (P.njobs == 1) {
+            jobs[0].large.resize(nlarge);
+            fill(jobs[0].large.data(), nullptr);
+        } else if (nlarge) {
+            std::vector<uint32_t> n_of(P.njobs, 0), at(nlarge);
+            for (uint64_t k = 0; k < nlarge; k++) at[k] = n_of[raw[k].job]++;
+            std::vector<voss_large_gap> all(nlarge);
+            std::vector<uint64_t> first(P.njobs + 1, 0);
+            for (uint32_t j = 0; j < P.njobs; j++) first[j + 1] = first[j] + n_of[j];
+            for (uint64_t k = 0; k < nlarge; k++) at[k] += (uint32_t)first[raw[k].job];
+            fill(all.data(), at.data());
+            for (uint32_t j = 0; j < P.njobs; j++)
+                if (n_of[j])
+                    jobs[j].large.assign(all.begin() + first[j], all.begin() + first[j + 1]);
         }
         const SegRec* rec = B->h_rec.p;
         if (counts) {
@@ -1149,8 +1195,9 @@ int execute(voss_ctx* c, Plan&& P, std::vector<JobHost>& jobs, ExecTimes* times,
     X.P = std::move(P);
     X.plan_rounds();
     int rc = X.run(use_graph && c->graph_mode, times);
-    if (rc) return rc;
-    return X.collect(jobs, counts);
+    if (!rc) rc = X.collect(jobs, counts);
+    c->desc_pool.swap(X.P.desc);  // memory for the next plan
+    return rc;
 }
 
 double now_s() {
@@ -1449,8 +1496,10 @@ int plain_args(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) {
     return set_device(c);
 }
 
-Plan plain_plan(const voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t s0, uint64_t s1) {
+Plan plain_plan(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t s0, uint64_t s1) {
     Plan P;
+    P.desc.swap(c->desc_pool);  // reuse the last plan's memory
+    P.desc.clear();
     P.njobs = 1;
     const uint32_t W = c->tune.window_bytes;
     // windows of the range plus one split per logical-segment edge
@@ -1511,8 +1560,10 @@ int check_interval_out(const voss_interval_result* out, size_t n) {
 }
 
 // Job i = interval i, each on its own window grid.
-Plan plan_batch(const voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n) {
+Plan plan_batch(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n) {
     Plan P;
+    P.desc.swap(c->desc_pool);
+    P.desc.clear();
     P.njobs = (uint32_t)n;
     const uint32_t W = c->tune.window_bytes;
     for (size_t i = 0; i < n; i++) {

@@ -107,6 +107,28 @@ __device__ __forceinline__ uint32_t first_pos(uint32_t p, uint32_t r,
     return pos;
 }
 
+// The sieve's per-(prime, window) setup without 32-bit multiplies or
+// divisions per class (those are quarter-rate on the VALU):
+//   floor(p*k/30) = pq*k + floor(pr*k/30)   (pq = p / 30, pr = p % 30,
+//   k = W30[j]), and floor(pr*k/30) = (pr*k*2185) >> 16 exactly for
+//   pr, k < 30 (24-bit multiply).  pqk = pq*k.
+__device__ __forceinline__ uint32_t class_off(uint32_t pqk, uint32_t pr, uint32_t k) {
+    return pqk + (__umul24(pr, k * 2185u) >> 16);
+}
+
+// first_pos from the class offset off = floor(p*W30[j]/30) < p and
+// r = w0 mod p < p: (off - r) mod p is the smaller of off - r and
+// off - r + p as unsigned 32-bit values.
+__device__ __forceinline__ uint32_t first_pos_off(uint32_t off, uint32_t p, uint32_t r,
+                                                  bool cls0, uint64_t w0) {
+    const uint32_t d = off - r;
+    uint32_t pos = min(d, d + p);
+    if (cls0 && w0 <= (uint64_t)off) pos += p;
+    return pos;
+}
+
+__device__ __forceinline__ uint32_t prime_mod30(uint32_t perm) { return (perm >> 24) & 31u; }
+
 // wheel bit of residue r (r coprime to 30), four bits per residue
 constexpr unsigned long long res_nibbles(int half) {
     unsigned long long v = 0;
@@ -335,7 +357,10 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     const int j = lane >> 3;
                     const uint32_t s = lane & 7;
                     const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
-                    const uint32_t b = first_pos(p, r, j, D.w0) + s * p;
+                    // p < tier_a_limit: 24-bit products
+                    const uint32_t k = w30_of((uint32_t)j);
+                    const uint32_t off = class_off(__umul24(p / 30u, k), prime_mod30(pr.perm), k);
+                    const uint32_t b = first_pos_off(off, p, r, j == 0, D.w0) + __umul24(s, p);
                     // stride 8p = 0 mod 4: one mask, and the word address
                     // b & ~3 advances exactly by the stride
                     const uint32_t stride = 8u * p;
@@ -354,10 +379,15 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                         const PrimeRec pr = a.primes[pi];
                         const uint32_t p = pr.p;
                         const uint32_t r = mod_u64(D.w0, p, pr.magic);
+                        const uint32_t pq = p / 30u, pm = prime_mod30(pr.perm);
 #pragma unroll 1
-                        for (int j = 0; j < 8; j++)
-                            cross_class(bm, first_pos(p, r, j, D.w0), p,
+                        for (int j = 0; j < 8; j++) {
+                            // p < len: 24-bit products
+                            const uint32_t k = w30_of((uint32_t)j);
+                            const uint32_t off = class_off(__umul24(pq, k), pm, k);
+                            cross_class(bm, first_pos_off(off, p, r, j == 0, D.w0), p,
                                         (pr.perm >> (3 * j)) & 7u, len);
+                        }
                     }
                 } else {
                     const uint32_t pi0 = b2 + (t - nA - nB1) * 256 + lane;
@@ -374,11 +404,25 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                         if (7ull * p >= 30ull * len) {
                             walk_prime(bm, p, pr[q].magic, D.w0, len);
                         } else {
-                            // at most one multiple per class in the window
+                            // at most one multiple per class in the window;
+                            // pq*W30[j] by additions (W30 steps 6,4,2,4,2,4,6)
                             const uint32_t r = mod_u64(D.w0, p, pr[q].magic);
+                            const uint32_t pq = p / 30u, pm = prime_mod30(pr[q].perm);
+                            uint32_t pq2 = pq + pq, pq4 = pq2 + pq2;
+                            // opaque to the compiler, which would otherwise
+                            // fold the additions back into multiplies
+                            asm volatile("" : "+v"(pq2), "+v"(pq4));
+                            const uint32_t pq6 = pq4 + pq2;
+                            uint32_t pqk = pq;
 #pragma unroll
                             for (int j = 0; j < 8; j++) {
-                                const uint32_t b = first_pos(p, r, j, D.w0);
+                                const uint32_t k = w30_of((uint32_t)j);
+                                if (j > 0) {
+                                    const uint32_t dk = k - w30_of((uint32_t)j - 1u);
+                                    pqk += dk == 6u ? pq6 : dk == 4u ? pq4 : pq2;
+                                }
+                                const uint32_t b =
+                                    first_pos_off(class_off(pqk, pm, k), p, r, j == 0, D.w0);
                                 if (b < len) clear_bit(bm, b, (pr[q].perm >> (3 * j)) & 7u);
                             }
                         }
@@ -467,7 +511,7 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             emit_at = a.emit_off[d] + woff + incl - cnt;
             __syncthreads();
         }
-        // walk: primes of the run in increasing order, word by word.  The
+        // walk: primes of the run in increasing order.  The
         // first prime of the run pairs with itself (gap 0, into the unused
         // bin 0), so the loop body has no first-prime case.
         uint32_t first = 0, prev = 0;
@@ -507,12 +551,18 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                 }
                 prevoff = off;
             };
-            for (; i < iend; i++) {
-                uint32_t w = bm[i];
-                if (!w) continue;
-                const uint32_t base = 120u * i;
-                do visit(w, base);
-                while (w);
+            // one iteration per prime, not per word: a wave runs as many
+            // iterations as its lane with the most primes (a per-word loop
+            // runs, per word, as many as the lane with the fullest word).
+            // cnt counts the run's set bits, so the refill never passes iend.
+            uint32_t w = i < iend ? bm[i] : 0u, base = 120u * i;
+            for (uint32_t k = 0; k < cnt; k++) {
+                while (w == 0u) {
+                    i++;
+                    w = bm[i];
+                    base += 120u;
+                }
+                visit(w, base);
             }
             prev = prev ? prevoff + 1u : 0u;
         }
@@ -634,9 +684,10 @@ __global__ void init_acc_kernel(JobAcc* acc, uint32_t njobs) {
 // per window on the bucket's cursor (the cursors start at 0).  Tasks are
 // taken from a device counter, largest first (the host orders them).  Two ways to enumerate the
 // composite multiples m*p (m >= 2, coprime to 30) of a prime in the tile:
-//   class mode  thread per (prime, residue class): first multiple of the
-//               class by Barrett reduction, then stride p bytes.  Best when
-//               p is below the tile length (several hits per class);
+//   class mode  thread per prime: one Barrett reduction, then the eight
+//               residue classes in turn (first multiple of the class, then
+//               stride p bytes).  Best when p is below the tile length
+//               (several hits per class);
 //   walk mode   thread per prime: one Barrett reduction gives the first
 //               multiple at or after the tile start, then the thread steps
 //               m -> m+1 (p/30 bytes and p%30 in the residue) and keeps the
@@ -649,37 +700,39 @@ __global__ void init_acc_kernel(JobAcc* acc, uint32_t njobs) {
 // bucket_cnt) still includes them and *overflow is set (the caller then
 // re-runs with exact counts), so no write ever leaves the bucket.
 
-// Calls f(byte offset in the tile, wheel bit) for every hit of work item k
-// of a tile starting at byte w0 (record pr already loaded: prime k in walk
-// mode, prime k/8 otherwise).
+// Walk mode: calls f(byte offset in the tile, wheel bit) for every hit of
+// prime pr in the tile starting at byte w0.
 template <typename F>
-__device__ __forceinline__ void bucket_hits(bool walk, uint64_t w0, const PrimeRec& pr, uint32_t k,
-                                            uint32_t span, F&& f) {
-    if (walk) {
-        const uint32_t p = pr.p;
-        const uint64_t X = 30ull * w0;  // first number of the tile
-        uint32_t x;                     // first multiple m*p >= X with m >= 2, minus X
-        if (X < 2ull * p) {
-            x = (uint32_t)(2ull * p - X);
-        } else {
-            const uint32_t r = mod_u64(X, p, pr.magic);
-            x = r ? p - r : 0u;
-        }
-        uint32_t b = x / 30u, rr = x - 30u * b;
-        const uint32_t pd = p / 30u, pm = p - 30u * pd;
-        while (b < span) {
-            if ((kCoprime30 >> rr) & 1u) f(b, wheel_bit_of_residue(rr));
-            b += pd;
-            rr += pm;
-            if (rr >= 30u) {
-                rr -= 30u;
-                b++;
-            }
-        }
+__device__ __forceinline__ void walk_hits(uint64_t w0, const PrimeRec& pr, uint32_t span, F&& f) {
+    const uint32_t p = pr.p;
+    const uint64_t X = 30ull * w0;  // first number of the tile
+    uint32_t x;                     // first multiple m*p >= X with m >= 2, minus X
+    if (X < 2ull * p) {
+        x = (uint32_t)(2ull * p - X);
     } else {
-        const int j = (int)(k & 7u);
-        const uint32_t p = pr.p;
-        const uint32_t r = mod_u64(w0, p, pr.magic);
+        const uint32_t r = mod_u64(X, p, pr.magic);
+        x = r ? p - r : 0u;
+    }
+    uint32_t b = x / 30u, rr = x - 30u * b;
+    const uint32_t pd = p / 30u, pm = p - 30u * pd;
+    while (b < span) {
+        if ((kCoprime30 >> rr) & 1u) f(b, wheel_bit_of_residue(rr));
+        b += pd;
+        rr += pm;
+        if (rr >= 30u) {
+            rr -= 30u;
+            b++;
+        }
+    }
+}
+
+// Class mode, the same for every hit.
+template <typename F>
+__device__ __forceinline__ void class_hits(uint64_t w0, const PrimeRec& pr, uint32_t span, F&& f) {
+    const uint32_t p = pr.p;
+    const uint32_t r = mod_u64(w0, p, pr.magic);
+#pragma unroll 1
+    for (int j = 0; j < 8; j++) {
         const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
         for (uint32_t b = first_pos(p, r, j, w0); b < span; b += p) f(b, bit);
     }
@@ -688,58 +741,56 @@ __device__ __forceinline__ void bucket_hits(bool walk, uint64_t w0, const PrimeR
 // FILL in class mode, four hits per step: the four slot reservations (LDS
 // atomics with return) and the four room reads are issued before any result
 // is used, so their latencies overlap instead of adding up per hit.
-__device__ __forceinline__ void bucket_fill_class(uint64_t w0, const PrimeRec& pr, uint32_t k,
-                                                  uint32_t span, uint32_t* cnt,
-                                                  const uint32_t* lim, uint32_t* entries,
-                                                  int log2w, uint32_t wmask) {
-    const int j = (int)(k & 7u);
+__device__ __forceinline__ void bucket_fill_class(uint64_t w0, const PrimeRec& pr, uint32_t span,
+                                                  uint32_t* cnt, const uint32_t* lim,
+                                                  uint32_t* entries, int log2w, uint32_t wmask) {
     const uint32_t p = pr.p;
     const uint32_t r = mod_u64(w0, p, pr.magic);
-    const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
-    uint32_t b = first_pos(p, r, j, w0);
-    while ((uint64_t)b + 3ull * p < span) {
-        uint32_t bb[4], sl[4], lm[4];
+#pragma unroll 1
+    for (int j = 0; j < 8; j++) {
+        const uint32_t bit = (pr.perm >> (3 * j)) & 7u;
+        uint32_t b = first_pos(p, r, j, w0);
+        while ((uint64_t)b + 3ull * p < span) {
+            uint32_t bb[4], sl[4], lm[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            bb[q] = b + (uint32_t)q * p;
-            sl[q] = atomicAdd(&cnt[bb[q] >> log2w], 1u);
+            for (int q = 0; q < 4; q++) {
+                bb[q] = b + (uint32_t)q * p;
+                sl[q] = atomicAdd(&cnt[bb[q] >> log2w], 1u);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) lm[q] = lim[bb[q] >> log2w];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (sl[q] < lm[q]) entries[sl[q]] = ((bb[q] & wmask) << 3) | bit;
+            b += 4u * p;
         }
-#pragma unroll
-        for (int q = 0; q < 4; q++) lm[q] = lim[bb[q] >> log2w];
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-            if (sl[q] < lm[q]) entries[sl[q]] = ((bb[q] & wmask) << 3) | bit;
-        b += 4u * p;
-    }
-    for (; b < span; b += p) {
-        const uint32_t w = b >> log2w;
-        const uint32_t slot = atomicAdd(&cnt[w], 1u);
-        if (slot < lim[w]) entries[slot] = ((b & wmask) << 3) | bit;
+        for (; b < span; b += p) {
+            const uint32_t w = b >> log2w;
+            const uint32_t slot = atomicAdd(&cnt[w], 1u);
+            if (slot < lim[w]) entries[slot] = ((b & wmask) << 3) | bit;
+        }
     }
 }
 
-// Every work item of the prime range [i0, i1) (8 per prime in class mode),
-// four per thread at a time: the four prime records are loaded before any
-// is used, so one load latency covers four items.
+// g(record) for every prime of [i0, i1), four per thread at a time: the four
+// prime records are loaded before any is used, so one load latency covers
+// four primes.
 template <typename G>
-__device__ __forceinline__ void bucket_sweep_items(uint32_t i0, uint32_t i1, bool walk,
-                                                   const PrimeRec* __restrict__ primes, G&& g) {
+__device__ __forceinline__ void bucket_sweep_primes(uint32_t i0, uint32_t i1,
+                                                    const PrimeRec* __restrict__ primes, G&& g) {
     if (i1 <= i0) return;
-    const uint32_t nthr = (i1 - i0) * (walk ? 1u : 8u);
+    const uint32_t n = i1 - i0;
     const uint32_t last = i1 - 1u;
-    for (uint32_t k0 = threadIdx.x; k0 < nthr; k0 += 4u * blockDim.x) {
+    for (uint32_t k0 = threadIdx.x; k0 < n; k0 += 4u * blockDim.x) {
         PrimeRec pr[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const uint32_t k = k0 + q * blockDim.x;
-            const uint32_t i = i0 + (walk ? k : k >> 3);
+            const uint32_t i = i0 + k0 + q * blockDim.x;
             pr[q] = primes[i < last ? i : last];
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint32_t k = k0 + q * blockDim.x;
-            if (k < nthr) g(pr[q], k);
-        }
+        for (int q = 0; q < 4; q++)
+            if (k0 + q * blockDim.x < n) g(pr[q]);
     }
 }
 
@@ -776,11 +827,11 @@ __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restri
         __syncthreads();
         if (FILL && T.shared) {
             // count, then reserve this task's slots in each window's bucket
-            bucket_sweep_items(T.i0, T.iw, false, primes, [&](const PrimeRec& pr, uint32_t k) {
-                bucket_hits(false, T.w0, pr, k, span, count);
+            bucket_sweep_primes(T.i0, T.iw, primes, [&](const PrimeRec& pr) {
+                class_hits(T.w0, pr, span, count);
             });
-            bucket_sweep_items(T.iw, T.i1, true, primes, [&](const PrimeRec& pr, uint32_t k) {
-                bucket_hits(true, T.w0, pr, k, span, count);
+            bucket_sweep_primes(T.iw, T.i1, primes, [&](const PrimeRec& pr) {
+                walk_hits(T.w0, pr, span, count);
             });
             __syncthreads();
             for (uint32_t i = tid; i < T.nwin; i += blockDim.x) {
@@ -793,18 +844,18 @@ __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restri
             __syncthreads();
         }
         if (!FILL) {
-            bucket_sweep_items(T.i0, T.iw, false, primes, [&](const PrimeRec& pr, uint32_t k) {
-                bucket_hits(false, T.w0, pr, k, span, count);
+            bucket_sweep_primes(T.i0, T.iw, primes, [&](const PrimeRec& pr) {
+                class_hits(T.w0, pr, span, count);
             });
-            bucket_sweep_items(T.iw, T.i1, true, primes, [&](const PrimeRec& pr, uint32_t k) {
-                bucket_hits(true, T.w0, pr, k, span, count);
+            bucket_sweep_primes(T.iw, T.i1, primes, [&](const PrimeRec& pr) {
+                walk_hits(T.w0, pr, span, count);
             });
         } else {
-            bucket_sweep_items(T.i0, T.iw, false, primes, [&](const PrimeRec& pr, uint32_t k) {
-                bucket_fill_class(T.w0, pr, k, span, cnt, lim, entries, log2w, wmask);
+            bucket_sweep_primes(T.i0, T.iw, primes, [&](const PrimeRec& pr) {
+                bucket_fill_class(T.w0, pr, span, cnt, lim, entries, log2w, wmask);
             });
-            bucket_sweep_items(T.iw, T.i1, true, primes, [&](const PrimeRec& pr, uint32_t k) {
-                bucket_hits(true, T.w0, pr, k, span, [&](uint32_t b, uint32_t bit) {
+            bucket_sweep_primes(T.iw, T.i1, primes, [&](const PrimeRec& pr) {
+                walk_hits(T.w0, pr, span, [&](uint32_t b, uint32_t bit) {
                     const uint32_t w = b >> log2w;
                     const uint32_t slot = atomicAdd(&cnt[w], 1u);
                     if (slot < lim[w]) entries[slot] = ((b & wmask) << 3) | bit;

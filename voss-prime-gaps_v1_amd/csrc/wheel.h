@@ -80,8 +80,8 @@ struct BucketTask {
 constexpr int kBucketTileMax = 256;  // windows per task (LDS counters)
 
 // Per base prime, one 16-byte record read with a single load: the prime,
-// the wheel bit of p*W30[j] for each residue class j (3 bits each) and the
-// Barrett constant floor(2^64 / p).
+// in perm the wheel bit of p*W30[j] for each residue class j (bits 3j..3j+2)
+// and p % 30 (bits 24..28), and the Barrett constant floor(2^64 / p).
 struct PrimeRec {
     uint32_t p;
     uint32_t perm;

@@ -40,10 +40,10 @@ LARGE_DTYPE = np.dtype([("position", "<u8"), ("gap", "<i4"), ("merit", "<f4")])
 
 def large_array(ptr, n):
     """View n voss_large_gap records at ptr as a numpy structured array (copy)."""
-    if n == 0:
-        return np.zeros(0, dtype=LARGE_DTYPE)
-    raw = (ctypes.c_char * (16 * n)).from_address(ctypes.addressof(ptr.contents))
-    return np.frombuffer(raw, dtype=LARGE_DTYPE).copy()
+    out = np.empty(n, dtype=LARGE_DTYPE)
+    if n:  # one memmove (a structured-dtype copy goes record by record)
+        ctypes.memmove(out.ctypes.data, ptr, 16 * n)
+    return out
 
 
 class Result(ctypes.Structure):
