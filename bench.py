@@ -44,7 +44,7 @@ LDS_PEAK_GBS = 128 * 256 * 2.4e9 / 1e9
 # runs (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, scaled by a 1 GiB copy kernel);
 # the commands and raw counters are in
 # profiles/sieve_kernel_plain_1e11_hbm_traffic.md.
-SIEVE_1E11_HBM_BYTES = (2 * 3965.4 + 17968.3) * 1024
+SIEVE_1E11_HBM_BYTES = (2 * 3938.6 + 18008.3) * 1024
 
 
 def model_lds_bytes(N):
@@ -339,8 +339,8 @@ def interval_leg(eng):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mr-count", type=int, default=10**7)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -354,7 +354,7 @@ def main():
                     help="run the batch leg with the default tuning")
     ap.add_argument("--legs", default="interval,batch,serve",
                     help="comma-separated legs after the headline: interval, batch, serve, "
-                         "plain1e14 (not default: about 35 s)")
+                         "plain1e12, plain1e13, plain1e14 (not default: 0.2 s, 2.7 s, 35 s)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -395,14 +395,19 @@ def main():
                                              mr_count=args.mr_count)
     if "serve" in want:
         legs["serve"] = serve_leg(eng, W)
-    if "plain1e14" in want:  # not in the default set: ~35 s
+    # plain runs at the reference's larger published scales (not in the
+    # default set: [1, 1e14] takes about 35 s); pi(N) from README.md:122-127
+    for e, pi, ref_s in ((12, 37607912018, 68.5), (13, 346065536839, 954.0),
+                         (14, 3204941750802, 21671.0)):
+        if f"plain1e{e}" not in want:
+            continue
         t = time.perf_counter()
-        r14 = eng.run_plain(10**14, SEG_NUM, large_cap=16)
+        rN = eng.run_plain(10**e, SEG_NUM, large_cap=16)
         dt = time.perf_counter() - t
-        assert r14["total_primes"] == 3204941750802, r14["total_primes"]
-        legs["plain_1e14"] = {"seconds": dt, "numbers_per_s": 1e14 / dt,
-                              "reference_seconds": 21671.0,  # README.md:127, A100-80GB
-                              "windows": r14["windows"], "large_gaps": r14["large_count"]}
+        assert rN["total_primes"] == pi, rN["total_primes"]
+        legs[f"plain_1e{e}"] = {"seconds": dt, "numbers_per_s": 10**e / dt,
+                                "reference_seconds": ref_s, "windows": rN["windows"],
+                                "large_gaps": rN["large_count"]}
 
     # Miller-Rabin tests/s on device-resident candidates (config 4's set)
     cand = W.mr_candidates(args.mr_count, ends)
@@ -464,6 +469,7 @@ def main():
                      "algorithmic_bytes": lds_bytes, "m_eff": m_eff,
                      "avg_kernel_ms": statistics.mean(sieve_ms)},
         "device_ms_per_step": statistics.mean(dev_ms),
+        "step_ms": [round(1e3 * x, 3) for x in step_s],
         "lds_probe": probe,
         "mr": mr,
         "legs": legs,

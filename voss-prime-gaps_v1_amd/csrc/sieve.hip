@@ -221,7 +221,9 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
     constexpr int W4 = WIN / 4;
     __shared__ __align__(16) uint32_t bm[W4];
     __shared__ uint32_t hist[kHistBins];
-    __shared__ uint32_t mq[64 * 8];  // q=30 rows interleaved: mq[8*(g/2) + row]
+    // q=30 rows interleaved: mq[8*(g/2) + row] for gaps g < 100; bin 0 takes
+    // the walk's gap-0 self-pairs and bin 50 its gaps >= 100 (both dropped)
+    __shared__ uint32_t mq[64 * 8];
     __shared__ uint32_t s_pat[kPresieveWordsPadded];
     __shared__ uint32_t tfirst[BLOCK];
     __shared__ uint32_t tlast[BLOCK];
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
         for (int i = tid; i < 8 * 64; i += BLOCK) {
             const uint32_t v = mq[i];
             mq[i] = 0;
-            if (v && i >= 8) {  // bin 0 collects the gap-0 dummies
+            if (v && i >= 8 && i < 8 * 50) {  // bins 1..49: gaps 2..98
                 const uint32_t row = (uint32_t)i & 7u, h = (uint32_t)i >> 3;
                 const unsigned long long v64 = v;
                 atomicAdd(&acc->modq[(2u + row) * kModqBins + h], v64);
@@ -542,9 +544,9 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     const uint32_t g = off - prevoff;
                     // byte address 4*(8*(g/2) + row) = 16*g + 4*row (g even):
                     // equal gaps of different rows fall in different banks.
-                    // Gaps >= 100 add into the unused bin 0 instead of taking
-                    // a branch around the atomic, and go to pair_wide.
-                    const uint32_t gn = g < 100u ? g : 0u;
+                    // Gaps >= 100 add into the unused bin 50 instead of
+                    // taking a branch around the atomic, and go to pair_wide.
+                    const uint32_t gn = min(g, 100u);
                     atomicAdd((uint32_t*)(mqb + rowb + 16u * gn), 1u);
                     if (g >= 100u) pair_wide(pc, prevoff, g);
                     rowb = (b & 7u) << 2;
