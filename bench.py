@@ -233,7 +233,7 @@ def batch_with_mr(eng, lo, hi, cand, rb, same, replay_s):
     try:
         vossgpu._check(L.voss_memcpy_h2d(eng._h, dn, cand.ctypes.data, n * 8))
         _, mr_ms = eng.mr_device(cand[: min(n, 10**6)])  # warm-up
-        verdict, mr_ms = eng.mr_device(cand, repeats=3)
+        verdict, mr_ms = eng.mr_device(cand, repeats=5)
         with eng.prepare_batch(lo, hi) as pb:
             pb.add_mr(dn, n, dv)
             pb.run(large_cap=64, raw=True)  # capture
@@ -411,7 +411,7 @@ def main():
 
     # Miller-Rabin tests/s on device-resident candidates (config 4's set)
     cand = W.mr_candidates(args.mr_count, ends)
-    verdict, mr_ms = eng.mr_device(cand, repeats=3)
+    verdict, mr_ms = eng.mr_device(cand, repeats=5)
     mr_rate = len(cand) / (statistics.median(mr_ms) / 1e3)
     eng.mr(cand[:1000])
     t = time.perf_counter()

@@ -44,3 +44,13 @@ def test_verify_gaps(engine, oracle):
     pa = np.array([p for p, _ in rows], dtype=np.uint64)
     g = np.array([x for _, x in rows], dtype=np.int32)
     assert np.array_equal(engine.verify_gaps(pa, g), oracle.verify_gaps(pa, g))
+
+
+def test_mr_near_powers_of_two(engine, oracle):
+    """Odd n around every 2^k: crosses the ranges in which the kernels
+    compute 2^64 mod n differently (below 2^13, up to 2^62, 2^62..2^63,
+    2^63 and above) and the 32-bit halves of the Montgomery square."""
+    ns = sorted({(1 << k) + d for k in range(2, 65) for d in range(-301, 302, 2)
+                 if 0 < (1 << k) + d < 2**64})
+    ns = np.array(ns, dtype=np.uint64)
+    assert np.array_equal(engine.mr(ns), oracle.is_prime_batch(ns))

@@ -26,7 +26,7 @@ __device__ __forceinline__ bool sprp_tail(const Mont& M, uint64_t x, int s) {
     const uint64_t mone = M.n - M.one;
     if (x == M.one || x == mone) return true;
     for (int j = 0; j < s - 1; j++) {
-        x = mont_mul(x, x, M);
+        x = mont_sqr(x, M);
         if (x == mone) return true;
         if (x == M.one) return false;
     }
@@ -40,7 +40,7 @@ __device__ __forceinline__ bool sprp2(const Mont& M, uint64_t d, int s) {
     uint64_t x = add_mod(M.one, M.one, M.n);  // 2 in Montgomery form
     const int top = 63 - __clzll((long long)d);
     for (int k = top - 1; k >= 0; k--) {
-        x = mont_mul(x, x, M);
+        x = mont_sqr(x, M);
         if ((d >> k) & 1) x = add_mod(x, x, M.n);
     }
     return sprp_tail(M, x, s);
@@ -64,9 +64,9 @@ __device__ __forceinline__ bool sprp_window(const Mont& M, uint64_t bm, uint64_t
     }
     uint64_t x = tbl[((uint32_t)(d >> (3 * ktop)) & 7u) * 256 + t];
     for (int k = ktop - 1; k >= 0; k--) {
-        x = mont_mul(x, x, M);
-        x = mont_mul(x, x, M);
-        x = mont_mul(x, x, M);
+        x = mont_sqr(x, M);
+        x = mont_sqr(x, M);
+        x = mont_sqr(x, M);
         x = mont_mul(x, tbl[((uint32_t)(d >> (3 * k)) & 7u) * 256 + t], M);
     }
     return sprp_tail(M, x, s);
@@ -74,25 +74,50 @@ __device__ __forceinline__ bool sprp_window(const Mont& M, uint64_t bm, uint64_t
 
 __constant__ uint32_t c_bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
 
-// 0 = composite, 1 = prime, 2 = needs the witness loop.
+// Divisibility of a 64-bit n by an odd constant b without a division:
+// b | n iff n * b^-1 mod 2^64 <= (2^64 - 1) / b (the multiples of b are
+// exactly the images of 0..(2^64-1)/b under multiplication by b^-1).
+constexpr uint64_t inv_mod64(uint64_t b) {
+    uint64_t x = b;  // correct to 3 bits for odd b
+    for (int i = 0; i < 5; i++) x *= 2 - b * x;
+    return x;
+}
+template <uint64_t B>
+__device__ __forceinline__ bool divisible(uint64_t n) {
+    return n * inv_mod64(B) <= ~0ull / B;
+}
+
+// 0 = composite, 1 = prime, 2 = needs the witness loop (the reference's
+// order: n < 2, n equal to a base, n divisible by a base).
 __device__ __forceinline__ int trial(uint64_t n) {
-    if (n < 2) return 0;
-#pragma unroll
-    for (int i = 0; i < 12; i++) {
-        const uint64_t b = c_bases[i];
-        if (n == b) return 1;
-        if (n % b == 0) return 0;
-    }
+    // the primes up to 37 as bits: 2,3,5,7,11,13,17,19,23,29,31,37
+    constexpr uint64_t kSmallPrimes = (1ull << 2) | (1ull << 3) | (1ull << 5) | (1ull << 7) |
+                                      (1ull << 11) | (1ull << 13) | (1ull << 17) |
+                                      (1ull << 19) | (1ull << 23) | (1ull << 29) |
+                                      (1ull << 31) | (1ull << 37);
+    if (n < 38) return (int)((kSmallPrimes >> n) & 1u);  // composites <= 37 have a base factor
+    if ((n & 1) == 0 || divisible<3>(n) || divisible<5>(n) || divisible<7>(n) ||
+        divisible<11>(n) || divisible<13>(n) || divisible<17>(n) || divisible<19>(n) ||
+        divisible<23>(n) || divisible<29>(n) || divisible<31>(n) || divisible<37>(n))
+        return 0;
     return 2;
 }
 
-__global__ __launch_bounds__(256) void mr_stage1(const uint64_t* __restrict__ cand,
-                                                 uint64_t count,
-                                                 uint8_t* __restrict__ verdict,
-                                                 uint32_t* __restrict__ surv,
-                                                 uint32_t* __restrict__ nsurv) {
+// Survivors are appended with one atomic per workgroup pass (wave counts by
+// ballot, their prefix in LDS): a single counter hit once per wave
+// serialises at the L2 and bounded this kernel.
+__global__ __launch_bounds__(1024) void mr_stage1(const uint64_t* __restrict__ cand,
+                                                  uint64_t count,
+                                                  uint8_t* __restrict__ verdict,
+                                                  uint32_t* __restrict__ surv,
+                                                  uint32_t* __restrict__ nsurv) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_base;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwaves = (int)(blockDim.x >> 6);
+    // the loop bound depends on blockIdx only, so every thread of the
+    // block takes every pass and reaches every barrier
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < count;
          base += stride) {
         const uint64_t i = base + threadIdx.x;
@@ -111,14 +136,21 @@ __global__ __launch_bounds__(256) void mr_stage1(const uint64_t* __restrict__ ca
             verdict[i] = (uint8_t)(v == 1);
         }
         const uint64_t mask = __ballot(more);
-        if (mask) {
-            const int leader = __ffsll((long long)mask) - 1;
-            uint32_t slot = 0;
-            if (lane == leader) slot = atomicAdd(nsurv, (uint32_t)__popcll(mask));
-            slot = __shfl(slot, leader);
-            if (more)
-                surv[slot + __popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)i;
+        if (lane == 0) s_wave[wave] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int w = 0; w < nwaves; w++) {
+                const uint32_t c = s_wave[w];
+                s_wave[w] = t;
+                t += c;
+            }
+            s_base = t ? atomicAdd(nsurv, t) : 0u;
         }
+        __syncthreads();
+        if (more)
+            surv[s_base + s_wave[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)i;
+        __syncthreads();  // s_wave and s_base are rewritten by the next pass
     }
 }
 
@@ -170,11 +202,17 @@ hipError_t launch_mr(const uint64_t* d_cand, uint64_t count, uint8_t* d_verdict,
     // a fill kernel, not hipMemsetAsync, so the sequence can be captured
     hipError_t e = launch_fill_u32(d_nsurv, 1, 0, s);
     if (e != hipSuccess) return e;
-    uint64_t blocks = (count + 255) / 256;
-    const uint64_t cap = (uint64_t)num_cus * 32;
-    if (blocks > cap) blocks = cap;
-    hipLaunchKernelGGL(mr_stage1, dim3((uint32_t)blocks), dim3(256), 0, s, d_cand,
+    // one thread per candidate (the kernels grid-stride only beyond 2^32
+    // threads): a capped grid with a few candidates per thread leaves its
+    // last pass half empty; stage 2 blocks beyond the survivor count exit
+    // at once
+    (void)num_cus;
+    uint64_t blocks1 = (count + 1023) / 1024;
+    if (blocks1 > (1u << 22)) blocks1 = 1u << 22;
+    hipLaunchKernelGGL(mr_stage1, dim3((uint32_t)blocks1), dim3(1024), 0, s, d_cand,
                        count, d_verdict, d_surv, d_nsurv);
+    uint64_t blocks = (count + 255) / 256;
+    if (blocks > (1u << 24)) blocks = 1u << 24;
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(mr_stage2, dim3((uint32_t)blocks), dim3(256), 0, s, d_cand,
                        d_surv, d_nsurv, d_verdict);
