@@ -92,7 +92,8 @@ struct ExecBufs {
     DevBuf<JobAcc> d_acc;
     DevBuf<LargeRaw> d_large;
     DevBuf<unsigned long long> d_flags;  // [0] large-gap records, [1] bucket overflow,
-                                         // [2] bucket task counter
+                                         // [2] bucket task counter, [3] sieve
+                                         // window counter (low 32 bits)
     DevBuf<BucketTask> d_btask;
     DevBuf<uint32_t> d_bcnt;
     DevBuf<uint32_t> d_bcur;
@@ -555,6 +556,7 @@ SieveArgs make_args(voss_ctx* c, ExecBufs& B, uint32_t ndesc, unsigned long long
     a.large = B.d_large.p;
     a.large_cap = large_cap;
     a.large_n = B.d_flags.p;
+    a.win_next = reinterpret_cast<uint32_t*>(B.d_flags.p + 3);
     return a;
 }
 
@@ -863,7 +865,7 @@ struct Executor {
         HIPCHK(B->d_rec.reserve(nd));
         HIPCHK(B->d_acc.reserve(nj));
         HIPCHK(B->d_large.reserve(lcap));
-        HIPCHK(B->d_flags.reserve(3));
+        HIPCHK(B->d_flags.reserve(4));
         HIPCHK(B->d_btask.reserve(ntask));
         HIPCHK(B->d_bcnt.reserve(nb_max));
         HIPCHK(B->d_bcur.reserve(nbkt));
@@ -896,6 +898,7 @@ struct Executor {
     hipEvent_t ev(size_t i) { return c->evs[i]; }
 
     uint32_t* task_counter() { return reinterpret_cast<uint32_t*>(B->d_flags.p + 2); }
+    uint32_t* win_counter() { return reinterpret_cast<uint32_t*>(B->d_flags.p + 3); }
 
     int ensure_events(size_t n) {
         while (c->evs.size() < n) {
@@ -949,6 +952,7 @@ struct Executor {
             }
             if (timed) HIPCHK(hipEventRecord(ev(4 * r + 1), s));
             const SieveArgs a = round_args(R);
+            HIPCHK(launch_fill_u32(win_counter(), 1, 0, s));
             HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, s));
             if (timed) HIPCHK(hipEventRecord(ev(4 * r + 2), s));
             HIPCHK(launch_stitch(a, s));
@@ -1021,6 +1025,7 @@ struct Executor {
             }
             if (timed) HIPCHK(hipEventRecord(ev(4 * r + 1), s));
             const SieveArgs a = round_args(R);
+            HIPCHK(launch_fill_u32(win_counter(), 1, 0, s));
             HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, s));
             if (timed) HIPCHK(hipEventRecord(ev(4 * r + 2), s));
             HIPCHK(launch_stitch(a, s));
@@ -1888,13 +1893,14 @@ int voss_sample_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, int32_t* gap_se
         HIPCHK(c->d_emit.reserve(std::max<unsigned long long>(total, 1)));
         HIPCHK(B.d_acc.reserve(1));
         HIPCHK(B.d_large.reserve(1));
-        HIPCHK(B.d_flags.reserve(3));
+        HIPCHK(B.d_flags.reserve(4));
         HIPCHK(hipMemcpyAsync(B.d_desc.p, emit_desc.data(), nd * sizeof(SegDesc),
                               hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->d_emit_off.p, emit_off.data(), nd * 8, hipMemcpyHostToDevice,
                               c->stream));
         HIPCHK(launch_init_acc(B.d_acc.p, 1, c->stream));
         SieveArgs a = make_args(c, B, nd, 1);
+        HIPCHK(launch_fill_u32(a.win_next, 1, 0, c->stream));
         HIPCHK(launch_sieve_emit(a, (int)W, (uint32_t)c->num_cus * c->tune.wg_per_cu, c->stream));
         std::vector<unsigned long long> primes(total);
         HIPCHK(hipMemcpyAsync(primes.data(), c->d_emit.p, total * 8, hipMemcpyDeviceToHost,

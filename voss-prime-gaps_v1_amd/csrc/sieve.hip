@@ -266,7 +266,12 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
         __syncthreads();
     };
 
-    for (uint32_t d = blockIdx.x; d < a.ndesc; d += gridDim.x) {
+    // windows are taken from a counter, not by a fixed stride: window costs
+    // vary (bucketed or not, partial, base-prime counts), and a fixed share
+    // per workgroup left a long tail in batches of mixed intervals.  A
+    // workgroup's windows still come in increasing order.
+    __shared__ uint32_t s_next;
+    for (uint32_t d = blockIdx.x; d < a.ndesc;) {
         const SegDesc D = a.desc[d];
         if (D.job != cur_job) {
             flush(cur_job);
@@ -617,7 +622,9 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             }
             a.rec[d] = rc;
         }
+        if (tid == 0) s_next = gridDim.x + atomicAdd(a.win_next, 1u);
         __syncthreads();
+        d = s_next;
     }
     flush(cur_job);
 }

@@ -31,6 +31,8 @@ struct SieveArgs {
     LargeRaw* large;
     unsigned long long large_cap;
     unsigned long long* large_n;
+    uint32_t* win_next;  // window counter, 0 at launch: a workgroup takes
+                         // window blockIdx.x, then gridDim.x + atomicAdd
 };
 
 hipError_t upload_sieve_constants(int device);
