@@ -6,8 +6,8 @@
 // (:145-179), thrust::sort (:522-524), mod4_count_kernel /
 // modq_count_kernel (:181-232) and gaps_w30_seg_kernel (:255-312).
 //
-// Design (one workgroup = one LDS window at a time, grid-stride over the
-// window descriptors):
+// Design (one workgroup = one LDS window at a time; a workgroup's first
+// window is blockIdx.x, its next ones come from a device counter):
 //   phase 0  the window's bitmap is initialised from 13 presieve pattern
 //            tables for primes 7..73 held in LDS, one AND per table per word;
 //   phase 1  every other base prime p with p*p <= window end crosses off
