@@ -170,8 +170,7 @@ bool merit_ge10(uint64_t p_before, uint32_t g) {
 // f(begin, end) over [0, n) split across up to 16 host threads when n is
 // large enough to pay for starting them.
 template <typename F>
-void parallel_for(size_t n, F&& f) {
-    const size_t kMinPerThread = 8192;
+void parallel_for(size_t n, F&& f, size_t kMinPerThread = 8192) {
     size_t nt = std::min<size_t>(16, std::max<unsigned>(1u, std::thread::hardware_concurrency()));
     nt = std::min(nt, std::max<size_t>(1, n / kMinPerThread));
     if (nt <= 1) {
@@ -357,10 +356,18 @@ struct JobHost {
     uint64_t large_count = 0;
     bool overflow = false;       // a gap >= 2048 (none exists below 2^64)
     std::vector<voss_large_gap> large;
-    JobHost() : hist2(kHistBins, 0), mq2(kModqRows * kModqBins, 0) {}
+    // the histograms are allocated on first use (init), so that a batch's
+    // thousands of jobs can be set up on several host threads
+    void init() {
+        if (hist2.empty()) {
+            hist2.assign(kHistBins, 0);
+            mq2.assign(kModqRows * kModqBins, 0);
+        }
+    }
 };
 
 void host_pair(JobHost& J, uint64_t p, uint64_t q, bool full) {
+    J.init();
     const uint64_t g = q - p;
     if (g == 0) return;
     if (g == 1) {
@@ -387,6 +394,7 @@ void host_pair(JobHost& J, uint64_t p, uint64_t q, bool full) {
 }
 
 void add_acc(JobHost& J, const JobAcc& A) {
+    J.init();
     for (int h = 1; h < kHistBins; h++) J.hist2[h] += (int64_t)A.hist[h];
     for (int i = 0; i < kModqRows * kModqBins; i++)
         if (i % kModqBins) J.mq2[i] += (int64_t)A.modq[i];
@@ -461,6 +469,7 @@ void sort_large(std::vector<voss_large_gap>& v) {
 
 // Histograms and large gaps into the reference's array layout.
 void write_result(JobHost& J, voss_result* out) {
+    J.init();
     std::memset(out->gaps, 0, sizeof(out->gaps));
     std::memset(out->modq_gap, 0, sizeof(out->modq_gap));
     out->gaps[1] = J.gap1;
@@ -621,6 +630,8 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
                d.nprimes > ibl;
     };
     size_t off = 0;
+    uint32_t memo_np = ~0u;
+    uint64_t memo_room = 0;
     while (off < P.desc.size()) {
         Round R;
         R.off = off;
@@ -631,8 +642,12 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
             const SegDesc& d = P.desc[end];
             if (bucketed(d)) {
                 any_bucket = true;
-                const uint64_t e = bucket_room(c->inv_prefix[d.nprimes] - c->inv_prefix[ibl], W,
-                                               c->tune.bucket_room_pct);
+                if (d.nprimes != memo_np) {  // consecutive windows mostly share it
+                    memo_np = d.nprimes;
+                    memo_room = bucket_room(c->inv_prefix[d.nprimes] - c->inv_prefix[ibl], W,
+                                            c->tune.bucket_room_pct);
+                }
+                const uint64_t e = memo_room;
                 if (end > off && est + e > kBucketEntryBudget * 15 / 16) break;
                 est += e;
             }
@@ -1124,10 +1139,11 @@ struct Executor {
     // records, and the pairs across round edges.
     int collect(std::vector<JobHost>& jobs, std::vector<uint64_t>* counts) {
         jobs.assign(P.njobs, JobHost());
-        for (uint32_t j = 0; j < P.njobs; j++) {
+        for (uint32_t j = 0; j < P.njobs; j++)
             if (B->h_acc.p[j].error) return fail(VOSS_ERANGE, "prime gap >= 2048 (beyond the device histogram)");
-            add_acc(jobs[j], B->h_acc.p[j]);
-        }
+        parallel_for(P.njobs, [&](size_t j0, size_t j1) {
+            for (size_t j = j0; j < j1; j++) add_acc(jobs[j], B->h_acc.p[j]);
+        }, 256);
         // large records with their merits (one log each, on several host
         // threads for long lists), straight into the job lists: a single
         // job's list in device order, several jobs' via per-job offsets
@@ -1292,6 +1308,7 @@ void finish_interval(JobHost& J, uint64_t lo, uint64_t hi, voss_interval_result*
     out->c1 = H.c1;
     out->c3 = H.c3;
     std::memcpy(out->modq, H.modq, sizeof(H.modq));
+    J.init();
     out->gaps[0] = (uint64_t)J.gap1;
     for (int h = 1; h < kHistBins; h++) out->gaps[h] = (uint64_t)J.hist2[h];
     for (int i = 0; i < kModqRows * kModqBins; i++) out->modq_gap[i] = (uint64_t)J.mq2[i];
@@ -1571,10 +1588,38 @@ Plan plan_batch(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n) {
     P.desc.clear();
     P.njobs = (uint32_t)n;
     const uint32_t W = c->tune.window_bytes;
-    for (size_t i = 0; i < n; i++) {
-        if (hi[i] <= 7) continue;
-        const uint64_t a = std::max<uint64_t>(lo[i], 7), b = hi[i] - 1;
-        plan_range(c, P, a, b, (uint32_t)i, 0, W, (a / 30) & ~4095ull);
+    auto plan_jobs = [&](Plan& Q, size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; i++) {
+            if (hi[i] <= 7) continue;
+            const uint64_t a = std::max<uint64_t>(lo[i], 7), b = hi[i] - 1;
+            plan_range(c, Q, a, b, (uint32_t)i, 0, W, (a / 30) & ~4095ull);
+        }
+    };
+    if (n < 2048) {
+        plan_jobs(P, 0, n);
+        return P;
+    }
+    // large batches: job ranges planned on several host threads, then
+    // concatenated in job order
+    std::mutex m;
+    std::vector<std::pair<size_t, Plan>> parts;
+    parallel_for(n, [&](size_t i0, size_t i1) {
+        Plan Q;
+        plan_jobs(Q, i0, i1);
+        std::lock_guard<std::mutex> g(m);
+        parts.emplace_back(i0, std::move(Q));
+    }, 256);
+    std::sort(parts.begin(), parts.end(),
+              [](const std::pair<size_t, Plan>& x, const std::pair<size_t, Plan>& y) {
+                  return x.first < y.first;
+              });
+    size_t total = 0;
+    for (const auto& q : parts) total += q.second.desc.size();
+    P.desc.resize(total);
+    size_t at = 0;
+    for (const auto& q : parts) {
+        std::memcpy(P.desc.data() + at, q.second.desc.data(), q.second.desc.size() * sizeof(SegDesc));
+        at += q.second.desc.size();
     }
     return P;
 }
@@ -1592,10 +1637,11 @@ void batch_stats(voss_result* stats, const ExecTimes& tm, double t0, uint64_t nw
 
 int finish_batch(std::vector<JobHost>& jobs, const uint64_t* lo, const uint64_t* hi, size_t n,
                  voss_interval_result* out) {
-    for (size_t i = 0; i < n; i++) {
-        finish_interval(jobs[i], lo[i], hi[i], &out[i]);
+    parallel_for(n, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; i++) finish_interval(jobs[i], lo[i], hi[i], &out[i]);
+    }, 256);
+    for (size_t i = 0; i < n; i++)
         if (jobs[i].overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");
-    }
     return VOSS_OK;
 }
 
@@ -1640,6 +1686,7 @@ int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) 
     // (5, first wheel prime) is the reference's host boundary gap of
     // segment 0 (:559-562): main histogram only
     if (J.count) host_pair(J, 5, J.first, false);
+    J.init();
     J.gap1 += 1;      // (2,3)  :634
     J.hist2[1] += 1;  // (3,5)  :635
     if (J.overflow) return fail(VOSS_ERANGE, "prime gap >= 2048");

@@ -327,13 +327,21 @@ class Engine:
 
     @staticmethod
     def _interval_results(n, large_cap):
-        res = (IntervalResult * n)()
-        lbuf = (LargeGap * max(large_cap * n, 1))()
-        base = ctypes.addressof(lbuf)
-        for k in range(n):
-            if large_cap:
-                res[k].large = ctypes.cast(base + 16 * large_cap * k, ctypes.POINTER(LargeGap))
-                res[k].large_capacity = large_cap
+        """n IntervalResult records with large_cap large-gap slots each.  The
+        memory comes uninitialised from numpy (the library clears every
+        record it fills) and the per-record pointers are written as one
+        strided array, not field by field from Python."""
+        size = ctypes.sizeof(IntervalResult)
+        rbuf = np.empty(max(n, 1) * size, dtype=np.uint8)
+        res = (IntervalResult * n).from_buffer(rbuf)
+        lbuf = np.empty(max(large_cap * n, 1) * ctypes.sizeof(LargeGap), dtype=np.uint8)
+        if n:
+            u64 = rbuf[:n * size].view(np.uint64).reshape(n, size // 8)
+            u64[:, IntervalResult.large_capacity.offset // 8] = large_cap
+            u64[:, IntervalResult.large.offset // 8] = (
+                (lbuf.ctypes.data + 16 * large_cap * np.arange(n, dtype=np.uint64))
+                if large_cap else 0)
+        res._keep = rbuf  # the records live in rbuf
         return res, lbuf
 
     @staticmethod
@@ -342,11 +350,27 @@ class Engine:
                 "sieve_ms": stats.sieve_ms, "windows": int(stats.windows),
                 "bucket_ms": stats.bucket_ms, "bucket_entries": int(stats.bucket_entries)}
 
-    def _batch(self, fn, lo, hi, large_cap, raw):
+    @staticmethod
+    def batch_results(n, large_cap=64):
+        """Caller-owned output records for n intervals (pass as out= to
+        run_batch / run_batch_plain / PreparedBatch.run to reuse them, as a
+        C caller reuses its voss_interval_result array)."""
+        return Engine._interval_results(n, large_cap)
+
+    @staticmethod
+    def _outputs(n, large_cap, out):
+        if out is None:
+            return Engine._interval_results(n, large_cap)
+        res, lbuf = out
+        if len(res) != n or (n and res[0].large_capacity != large_cap):
+            raise ValueError("out= holds records of another batch shape")
+        return res, lbuf
+
+    def _batch(self, fn, lo, hi, large_cap, raw, out=None):
         lo = np.ascontiguousarray(lo, dtype=np.uint64)
         hi = np.ascontiguousarray(hi, dtype=np.uint64)
         n = lo.shape[0]
-        res, lbuf = self._interval_results(n, large_cap)
+        res, lbuf = self._outputs(n, large_cap, out)
         stats = Result()
         _check(fn(self._h, lo.ctypes.data, hi.ctypes.data, n, ctypes.cast(res, ctypes.c_void_p),
                   ctypes.byref(stats)))
@@ -364,14 +388,15 @@ class Engine:
         replayed by every later run."""
         return PreparedBatch(self, lo, hi)
 
-    def run_batch(self, lo, hi, large_cap=64, raw=False):
+    def run_batch(self, lo, hi, large_cap=64, raw=False, out=None):
         """All intervals in one persistent launch.  Returns (results, stats);
-        raw=True returns the ctypes IntervalResult array instead of dicts."""
-        return self._batch(lib().voss_run_batch, lo, hi, large_cap, raw)
+        raw=True returns the ctypes IntervalResult array instead of dicts.
+        out: records from batch_results to fill instead of new ones."""
+        return self._batch(lib().voss_run_batch, lo, hi, large_cap, raw, out)
 
-    def run_batch_plain(self, lo, hi, large_cap=64, raw=False):
+    def run_batch_plain(self, lo, hi, large_cap=64, raw=False, out=None):
         """One launch sequence per interval (the comparison form)."""
-        return self._batch(lib().voss_run_batch_plain, lo, hi, large_cap, raw)
+        return self._batch(lib().voss_run_batch_plain, lo, hi, large_cap, raw, out)
 
     def mr(self, ns):
         ns = np.ascontiguousarray(ns, dtype=np.uint64)
@@ -427,9 +452,9 @@ class PreparedBatch:
         _check(lib().voss_batch_prepare(engine._h, self.lo.ctypes.data, self.hi.ctypes.data,
                                         self.lo.shape[0], ctypes.byref(self._b)))
 
-    def run(self, large_cap=64, raw=False):
+    def run(self, large_cap=64, raw=False, out=None):
         n = self.lo.shape[0]
-        res, lbuf = Engine._interval_results(n, large_cap)
+        res, lbuf = Engine._outputs(n, large_cap, out)
         stats = Result()
         _check(lib().voss_batch_run(self._b, ctypes.cast(res, ctypes.c_void_p), ctypes.byref(stats)))
         st = Engine._stats(stats)
