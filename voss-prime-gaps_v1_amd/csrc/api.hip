@@ -184,6 +184,40 @@ void parallel_for(size_t n, F&& f, size_t kMinPerThread = 8192) {
     for (auto& x : th) x.join();
 }
 
+// An allocator whose resize() leaves new elements default-initialised
+// (uninitialised for plain structs), so that a window list of 1.2 GB is not
+// zero-filled before the planner writes every element.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <typename U>
+    NoInitAlloc(const NoInitAlloc<U>&) {}
+    template <typename U>
+    void construct(U* p) {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <typename U, typename... Args>
+    void construct(U* p, Args&&... args) {
+        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+    }
+};
+using DescVec = std::vector<SegDesc, NoInitAlloc<SegDesc>>;
+using LargeVec = std::vector<voss_large_gap, NoInitAlloc<voss_large_gap>>;
+
+// memcpy on several host threads for large blocks (pinned staging of the
+// window descriptors: 1.2 GB for a plain run to 1e14).
+void par_copy(void* dst, const void* src, size_t bytes) {
+    const size_t kChunk = 1u << 22;
+    parallel_for((bytes + kChunk - 1) / kChunk, [&](size_t c0, size_t c1) {
+        const size_t b0 = c0 * kChunk, b1 = std::min(bytes, c1 * kChunk);
+        std::memcpy(static_cast<char*>(dst) + b0, static_cast<const char*>(src) + b0, b1 - b0);
+    }, 2);
+}
+
 }  // namespace
 
 struct voss_ctx {
@@ -212,7 +246,7 @@ struct voss_ctx {
     ExecBufs eb;                  // buffers of the context's own runs
     // window list of the last plain run, kept so the next plan reuses its
     // memory (a fresh 1 MB vector per call costs page faults)
-    std::vector<SegDesc> desc_pool;
+    DescVec desc_pool;
     std::vector<hipEvent_t> evs;  // per-round timing events
     int graph_mode = 1;           // batched runs as one captured hipGraph
 
@@ -355,7 +389,7 @@ struct JobHost {
     uint64_t first = 0, last = 0;
     uint64_t large_count = 0;
     bool overflow = false;       // a gap >= 2048 (none exists below 2^64)
-    std::vector<voss_large_gap> large;
+    LargeVec large;
     // the histograms are allocated on first use (init), so that a batch's
     // thousands of jobs can be set up on several host threads
     void init() {
@@ -415,18 +449,18 @@ void add_acc(JobHost& J, const JobAcc& A) {
 // radix sort (11-bit digits, digits that never vary skipped) of the
 // inverted merit bits carrying the list index, then runs of equal merit
 // sorted by position.
-void sort_large(std::vector<voss_large_gap>& v) {
+void sort_large(LargeVec& v) {
     const size_t n = v.size();
     if (n < 2) return;
     // scratch kept per thread for lists up to 2^20 records (fresh multi-MB
     // vectors per call cost page faults), local beyond
     static thread_local std::vector<uint32_t> keep_u32[4];
-    static thread_local std::vector<voss_large_gap> keep_out;
+    static thread_local LargeVec keep_out;
     std::vector<uint32_t> own_u32[4];
-    std::vector<voss_large_gap> own_out;
+    LargeVec own_out;
     const bool keep = n <= (1u << 20);
     std::vector<uint32_t>* u = keep ? keep_u32 : own_u32;
-    std::vector<voss_large_gap>& out = keep ? keep_out : own_out;
+    LargeVec& out = keep ? keep_out : own_out;
     for (int q = 0; q < 4; q++) u[q].resize(n);
     uint32_t *key = u[0].data(), *idx = u[1].data(), *key2 = u[2].data(), *idx2 = u[3].data();
     uint32_t all_or = 0, all_and = ~0u;
@@ -467,6 +501,28 @@ void sort_large(std::vector<voss_large_gap>& v) {
     v.swap(out);
 }
 
+// The canonical order of sort_large as a comparison.
+inline bool large_before(const voss_large_gap& x, const voss_large_gap& y) {
+    return x.merit > y.merit || (x.merit == y.merit && x.position < y.position);
+}
+
+// The k first records of the canonical order into dst (k <= v.size()):
+// each host thread selects its chunk's best k, then the best k of those.
+// For long lists of which the caller keeps a few (a plain run to 1e14 has
+// 4.2e7 candidates).
+void top_large(const LargeVec& v, size_t k, voss_large_gap* dst) {
+    std::mutex m;
+    std::vector<voss_large_gap> best;
+    parallel_for(v.size(), [&](size_t i0, size_t i1) {
+        std::vector<voss_large_gap> part(std::min(k, i1 - i0));
+        std::partial_sort_copy(v.begin() + i0, v.begin() + i1, part.begin(), part.end(),
+                               large_before);
+        std::lock_guard<std::mutex> g(m);
+        best.insert(best.end(), part.begin(), part.end());
+    }, 1u << 16);
+    std::partial_sort_copy(best.begin(), best.end(), dst, dst + k, large_before);
+}
+
 // Histograms and large gaps into the reference's array layout.
 void write_result(JobHost& J, voss_result* out) {
     J.init();
@@ -478,11 +534,14 @@ void write_result(JobHost& J, voss_result* out) {
         for (int h = 1; h < kModqBins; h++)
             out->modq_gap[r * VOSS_MODQ_MAX_GAP + 2 * h] = J.mq2[r * kModqBins + h];
     out->large_count = J.large_count;
-    sort_large(J.large);
-    if (out->large && out->large_capacity) {
-        const size_t n = std::min<size_t>(J.large.size(), out->large_capacity);
-        std::memcpy(out->large, J.large.data(), n * sizeof(voss_large_gap));
+    if (!out->large || !out->large_capacity) return;
+    const size_t n = std::min<size_t>(J.large.size(), out->large_capacity);
+    if (J.large.size() > (1u << 16) && n < J.large.size() / 4) {
+        top_large(J.large, n, out->large);  // a few of many: select, not sort
+        return;
     }
+    sort_large(J.large);
+    std::memcpy(out->large, J.large.data(), n * sizeof(voss_large_gap));
 }
 
 // Wheel-prime contribution to the residue counters (slot layout of the
@@ -495,15 +554,24 @@ void wheel_modq(const JobHost& J, uint64_t* modq) {
 }
 
 struct Plan {
-    std::vector<SegDesc> desc;
+    DescVec desc;
     uint32_t njobs = 0;
 };
 
-// Windows of the grid w0 = k*W covering numbers [a, b] (a >= 7).  The
+// Windows of the grid w0 = k*W covering numbers [a, b] (a >= 7): k from
+// ka = (a/30 - origin)/W to kb = (b/30 - origin)/W, each window non-empty
+// (30*w0 <= b and 30*(w0+W) > a for every such k).
+size_t plan_count(uint64_t a, uint64_t b, uint32_t W, uint64_t grid_origin) {
+    if (a < 7) a = 7;
+    if (a > b) return 0;
+    return (size_t)((b / 30 - grid_origin) / W - (a / 30 - grid_origin) / W + 1);
+}
+
+// Writes the plan_count(a, b, W, origin) windows of [a, b] to out.  The
 // base-prime count of consecutive windows only grows, so it is advanced
 // with one pointer instead of a search per window.
-void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job,
-                uint32_t lseg, uint32_t W, uint64_t grid_origin) {
+void plan_fill(const voss_ctx* c, SegDesc* out, uint64_t a, uint64_t b, uint32_t job,
+               uint32_t lseg, uint32_t W, uint64_t grid_origin) {
     if (a < 7) a = 7;
     if (a > b) return;
     const uint64_t ka = (a / 30 - grid_origin) / W, kb = (b / 30 - grid_origin) / W;
@@ -515,10 +583,9 @@ void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job
         const uint64_t w0 = grid_origin + k * W;
         const uint64_t lo = std::max<uint64_t>(a, 30 * w0);
         const uint64_t hi = std::min<uint64_t>(b, 30 * (w0 + W) - 1);
-        if (lo > hi) continue;
         uint64_t need = hi / 30 - w0 + 1;
         need = (need + 4095) / 4096 * 4096;
-        SegDesc d;
+        SegDesc& d = *out++;
         d.w0 = w0;
         d.n_lo = lo;
         d.n_hi = hi;
@@ -533,8 +600,16 @@ void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job
         d.ibig = d.len == W ? ibig_w
                             : (uint32_t)(std::lower_bound(c->primes.begin(), c->primes.end(),
                                                           d.len) - c->primes.begin());
-        P.desc.push_back(d);
     }
+}
+
+// Appends the windows of [a, b] to P.
+void plan_range(const voss_ctx* c, Plan& P, uint64_t a, uint64_t b, uint32_t job,
+                uint32_t lseg, uint32_t W, uint64_t grid_origin) {
+    const size_t n = plan_count(a, b, W, grid_origin), at = P.desc.size();
+    if (n == 0) return;
+    P.desc.resize(at + n);
+    plan_fill(c, P.desc.data() + at, a, b, job, lseg, W, grid_origin);
 }
 
 SieveArgs make_args(voss_ctx* c, ExecBufs& B, uint32_t ndesc, unsigned long long large_cap) {
@@ -600,7 +675,7 @@ uint32_t bucket_room(double s, uint32_t W, uint32_t pct) {
 // bucket_min_windows windows get a bucket for the primes >= bucket_limit.
 struct Round {
     size_t off = 0, nd = 0;        // plan windows [off, off + nd)
-    std::vector<SegDesc> desc;     // with bucket / nprimes set for this round
+    DescVec desc;                  // with bucket / nprimes set for this round
     std::vector<BucketTask> tasks;
     uint32_t nbuckets = 0;
     std::vector<uint32_t> room;    // entries per bucket
@@ -629,12 +704,16 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
         return bl != 0xFFFFFFFFu && job_nwin[d.job] >= c->tune.bucket_min_windows &&
                d.nprimes > ibl;
     };
+    // pass 1: round boundaries (the bucket budget is a running sum)
+    struct Span {
+        size_t off, end;
+        bool any_bucket;
+    };
+    std::vector<Span> spans;
     size_t off = 0;
     uint32_t memo_np = ~0u;
     uint64_t memo_room = 0;
     while (off < P.desc.size()) {
-        Round R;
-        R.off = off;
         uint64_t est = 0;
         size_t end = off;
         bool any_bucket = false;
@@ -653,102 +732,108 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
             }
             end++;
         }
-        R.nd = end - off;
-        if (!any_bucket) {  // the plan's windows as they are (R.desc stays empty)
-            rounds.push_back(std::move(R));
-            off = end;
-            continue;
-        }
-        R.desc.assign(P.desc.begin() + off, P.desc.begin() + end);
-        std::vector<std::pair<double, BucketTask>> work;  // tasks with their estimated cost
-        // buckets: consecutive windows of one job with the same w0 share one
-        size_t i = 0;
-        while (i < R.nd) {
-            SegDesc& d = R.desc[i];
-            if (!bucketed(d)) { i++; continue; }
-            // a run of this job's windows (on its grid, step W)
-            size_t j = i;
-            uint32_t nb0 = R.nbuckets;
-            uint64_t prev_w0 = ~0ull;
-            uint32_t np_max = 0;
-            std::vector<uint32_t> np_of;  // per bucket of the run: primes its windows need
-            while (j < R.nd && R.desc[j].job == d.job && bucketed(R.desc[j])) {
-                SegDesc& e = R.desc[j];
-                if (e.w0 != prev_w0) {
-                    if (prev_w0 != ~0ull && e.w0 != prev_w0 + W) break;
-                    R.nbuckets++;
-                    prev_w0 = e.w0;
-                    np_of.push_back(0);
-                }
-                e.bucket = R.nbuckets - 1;
-                np_max = std::max(np_max, e.nprimes);
-                np_of.back() = std::max(np_of.back(), e.nprimes);
-                e.nprimes = (uint32_t)ibl;
-                j++;
-            }
-            const uint32_t nwin = R.nbuckets - nb0;
-            const uint32_t room = bucket_room(c->inv_prefix[np_max] - c->inv_prefix[ibl], W,
-                                             c->tune.bucket_room_pct);
-            R.room.insert(R.room.end(), nwin, room);
-            R.room_total += (uint64_t)room * nwin;
-            for (uint32_t t0 = 0; t0 < nwin; t0 += kBucketTileMax) {
-                const uint32_t tn = std::min<uint32_t>(kBucketTileMax, nwin - t0);
-                const uint32_t np_t = *std::max_element(np_of.begin() + t0, np_of.begin() + t0 + tn);
-                // primes beyond 1.5x the tile length: walk mode
-                const uint64_t walk_from = std::min<uint64_t>(3ull * tn * W / 2, 0xFFFFFFFFu);
-                const size_t iw = std::min<size_t>(
-                    np_t, std::max<size_t>(ibl, (size_t)(std::lower_bound(
-                                                             c->primes.begin(), c->primes.end(),
-                                                             (uint32_t)walk_from) -
-                                                         c->primes.begin())));
-                BucketTask T;
-                T.w0 = d.w0 + (uint64_t)t0 * W;
-                T.b0 = nb0 + t0;
-                T.nwin = tn;
-                T.i0 = (uint32_t)ibl;
-                T.iw = (uint32_t)iw;
-                T.i1 = np_t;
-                T.shared = 0;
-                work.push_back({bucket_task_cost(c, T, W), T});
-            }
-            i = j;
-        }
-        // a tile with more work than a workgroup's fair share is split into
-        // prime chunks of about a quarter of that share (shared tasks)
-        double total = 0;
-        for (const auto& w : work) total += w.first;
-        const double share = total / (2.0 * std::max(c->num_cus, 1));  // two workgroups per CU
-        std::vector<std::pair<double, BucketTask>> split;
-        for (const auto& w : work) {
-            if (w.first <= share) {
-                split.push_back(w);
-                continue;
-            }
-            const BucketTask& T = w.second;
-            const uint32_t k = (uint32_t)std::min(256.0, std::ceil(4.0 * w.first / share));
-            for (int walk = 0; walk < 2; walk++) {
-                const uint32_t a0 = walk ? T.iw : T.i0, a1 = walk ? T.i1 : T.iw;
-                if (a1 <= a0) continue;
-                const uint32_t step = std::max<uint32_t>(1, (a1 - a0 + k - 1) / k);
-                for (uint32_t x = a0; x < a1; x += step) {
-                    BucketTask S = T;
-                    S.i0 = x;
-                    S.iw = walk ? x : std::min(a1, x + step);
-                    S.i1 = std::min(a1, x + step);
-                    S.shared = 1;
-                    split.push_back({bucket_task_cost(c, S, W), S});
-                }
-            }
-        }
-        work.swap(split);
-        // largest tasks first, so the last ones to start are the short ones
-        std::stable_sort(work.begin(), work.end(),
-                         [](const std::pair<double, BucketTask>& x,
-                            const std::pair<double, BucketTask>& y) { return x.first > y.first; });
-        for (const auto& w : work) R.tasks.push_back(w.second);
-        rounds.push_back(std::move(R));
+        spans.push_back({off, end, any_bucket});
         off = end;
     }
+    // pass 2: each round's buckets and tasks, rounds on several host threads
+    rounds.assign(spans.size(), Round());
+    parallel_for(spans.size(), [&](size_t r0, size_t r1) {
+        for (size_t ri = r0; ri < r1; ri++) {
+            Round& R = rounds[ri];
+            R.off = spans[ri].off;
+            const size_t end = spans[ri].end;
+            R.nd = end - R.off;
+            if (!spans[ri].any_bucket) continue;  // the plan's windows as they are (R.desc empty)
+            const size_t off = R.off;
+            R.desc.assign(P.desc.begin() + off, P.desc.begin() + end);
+            std::vector<std::pair<double, BucketTask>> work;  // tasks with their estimated cost
+            // buckets: consecutive windows of one job with the same w0 share one
+            size_t i = 0;
+            while (i < R.nd) {
+                SegDesc& d = R.desc[i];
+                if (!bucketed(d)) { i++; continue; }
+                // a run of this job's windows (on its grid, step W)
+                size_t j = i;
+                uint32_t nb0 = R.nbuckets;
+                uint64_t prev_w0 = ~0ull;
+                uint32_t np_max = 0;
+                std::vector<uint32_t> np_of;  // per bucket of the run: primes its windows need
+                while (j < R.nd && R.desc[j].job == d.job && bucketed(R.desc[j])) {
+                    SegDesc& e = R.desc[j];
+                    if (e.w0 != prev_w0) {
+                        if (prev_w0 != ~0ull && e.w0 != prev_w0 + W) break;
+                        R.nbuckets++;
+                        prev_w0 = e.w0;
+                        np_of.push_back(0);
+                    }
+                    e.bucket = R.nbuckets - 1;
+                    np_max = std::max(np_max, e.nprimes);
+                    np_of.back() = std::max(np_of.back(), e.nprimes);
+                    e.nprimes = (uint32_t)ibl;
+                    j++;
+                }
+                const uint32_t nwin = R.nbuckets - nb0;
+                const uint32_t room = bucket_room(c->inv_prefix[np_max] - c->inv_prefix[ibl], W,
+                                                 c->tune.bucket_room_pct);
+                R.room.insert(R.room.end(), nwin, room);
+                R.room_total += (uint64_t)room * nwin;
+                for (uint32_t t0 = 0; t0 < nwin; t0 += kBucketTileMax) {
+                    const uint32_t tn = std::min<uint32_t>(kBucketTileMax, nwin - t0);
+                    const uint32_t np_t = *std::max_element(np_of.begin() + t0, np_of.begin() + t0 + tn);
+                    // primes beyond 1.5x the tile length: walk mode
+                    const uint64_t walk_from = std::min<uint64_t>(3ull * tn * W / 2, 0xFFFFFFFFu);
+                    const size_t iw = std::min<size_t>(
+                        np_t, std::max<size_t>(ibl, (size_t)(std::lower_bound(
+                                                                 c->primes.begin(), c->primes.end(),
+                                                                 (uint32_t)walk_from) -
+                                                             c->primes.begin())));
+                    BucketTask T;
+                    T.w0 = d.w0 + (uint64_t)t0 * W;
+                    T.b0 = nb0 + t0;
+                    T.nwin = tn;
+                    T.i0 = (uint32_t)ibl;
+                    T.iw = (uint32_t)iw;
+                    T.i1 = np_t;
+                    T.shared = 0;
+                    work.push_back({bucket_task_cost(c, T, W), T});
+                }
+                i = j;
+            }
+            // a tile with more work than a workgroup's fair share is split into
+            // prime chunks of about a quarter of that share (shared tasks)
+            double total = 0;
+            for (const auto& w : work) total += w.first;
+            const double share = total / (2.0 * std::max(c->num_cus, 1));  // two workgroups per CU
+            std::vector<std::pair<double, BucketTask>> split;
+            for (const auto& w : work) {
+                if (w.first <= share) {
+                    split.push_back(w);
+                    continue;
+                }
+                const BucketTask& T = w.second;
+                const uint32_t k = (uint32_t)std::min(256.0, std::ceil(4.0 * w.first / share));
+                for (int walk = 0; walk < 2; walk++) {
+                    const uint32_t a0 = walk ? T.iw : T.i0, a1 = walk ? T.i1 : T.iw;
+                    if (a1 <= a0) continue;
+                    const uint32_t step = std::max<uint32_t>(1, (a1 - a0 + k - 1) / k);
+                    for (uint32_t x = a0; x < a1; x += step) {
+                        BucketTask S = T;
+                        S.i0 = x;
+                        S.iw = walk ? x : std::min(a1, x + step);
+                        S.i1 = std::min(a1, x + step);
+                        S.shared = 1;
+                        split.push_back({bucket_task_cost(c, S, W), S});
+                    }
+                }
+            }
+            work.swap(split);
+            // largest tasks first, so the last ones to start are the short ones
+            std::stable_sort(work.begin(), work.end(),
+                             [](const std::pair<double, BucketTask>& x,
+                                const std::pair<double, BucketTask>& y) { return x.first > y.first; });
+            for (const auto& w : work) R.tasks.push_back(w.second);
+        }
+    }, 1);
 }
 
 int log2_window(const voss_ctx* c) {
@@ -861,10 +946,13 @@ struct Executor {
             // room for every large-gap candidate: about 1.5e-5 of the primes
             // at 1e11..1e14 (8.9e-6 at 1e11, 1.3e-5 at 1e13); 2e-5 of a
             // prime-count estimate avoids a re-run for overflow in practice
-            double primes = 0;  // every 16th window, scaled
-            for (size_t i = 0; i < P.desc.size(); i += 16) {
+            // (a sample of at most ~65536 windows, scaled)
+            const size_t step = std::max<size_t>(16, P.desc.size() >> 16);
+            double primes = 0;
+            for (size_t i = 0; i < P.desc.size(); i += step) {
                 const SegDesc& d = P.desc[i];
-                primes += 16.0 * (double)(d.n_hi - d.n_lo + 1) / std::log((double)d.n_hi + 3.0);
+                primes += (double)step * (double)(d.n_hi - d.n_lo + 1) /
+                          std::log((double)d.n_hi + 3.0);
             }
             lcap = std::max<uint64_t>({B->d_large.cap, 1u << 20, (uint64_t)(2e-5 * primes)});
         }
@@ -894,8 +982,8 @@ struct Executor {
         HIPCHK(B->h_acc.reserve(nj));
         HIPCHK(B->h_flags.reserve(2));
         for (const Round& R : rounds) {
-            std::memcpy(B->h_desc.p + R.off, R.desc.empty() ? P.desc.data() + R.off : R.desc.data(),
-                        R.nd * sizeof(SegDesc));
+            par_copy(B->h_desc.p + R.off, R.desc.empty() ? P.desc.data() + R.off : R.desc.data(),
+                     R.nd * sizeof(SegDesc));
             if (!R.tasks.empty())
                 std::memcpy(B->h_btask.p + R.task_off, R.tasks.data(),
                             R.tasks.size() * sizeof(BucketTask));
@@ -1163,7 +1251,7 @@ struct Executor {
         } else if (nlarge) {
             std::vector<uint32_t> n_of(P.njobs, 0), at(nlarge);
             for (uint64_t k = 0; k < nlarge; k++) at[k] = n_of[raw[k].job]++;
-            std::vector<voss_large_gap> all(nlarge);
+            LargeVec all(nlarge);
             std::vector<uint64_t> first(P.njobs + 1, 0);
             for (uint32_t j = 0; j < P.njobs; j++) first[j + 1] = first[j] + n_of[j];
             for (uint64_t k = 0; k < nlarge; k++) at[k] += (uint32_t)first[raw[k].job];
@@ -1175,12 +1263,34 @@ struct Executor {
         const SegRec* rec = B->h_rec.p;
         if (counts) {
             counts->resize(P.desc.size());
-            for (size_t i = 0; i < P.desc.size(); i++) (*counts)[i] = rec[i].count;
+            parallel_for(P.desc.size(), [&](size_t i0, size_t i1) {
+                for (size_t i = i0; i < i1; i++) (*counts)[i] = rec[i].count;
+            }, 1u << 18);
         }
         // round edge: pair the last prime of a job before the round with the
         // first non-empty window of the same job in the round (nothing to do
         // for a single round: the stitch kernel joined all its windows)
         if (rounds.size() < 2) return VOSS_OK;
+        if (P.njobs == 1) {
+            // one job: only the windows next to each edge matter, found by
+            // scanning from the edge (plain runs to 1e14 have 2.5e7 windows)
+            bool has = false;
+            uint64_t last = 0;
+            uint32_t lseg = 0;
+            for (const Round& R : rounds) {
+                size_t i = R.off;
+                const size_t e = R.off + R.nd;
+                while (i < e && rec[i].count == 0) i++;
+                if (i == e) continue;
+                if (has) host_pair(jobs[0], last, rec[i].first, lseg == P.desc[i].lseg);
+                size_t k = e - 1;
+                while (rec[k].count == 0) k--;  // stops at i at the latest
+                has = true;
+                last = rec[k].last;
+                lseg = P.desc[k].lseg;
+            }
+            return VOSS_OK;
+        }
         std::vector<uint64_t> carry_last(P.njobs, 0);
         std::vector<uint32_t> carry_lseg(P.njobs, 0);
         std::vector<uint8_t> carry_has(P.njobs, 0), seen(P.njobs, 0);
@@ -1524,14 +1634,30 @@ Plan plain_plan(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t s0, uint64_t
     P.desc.clear();
     P.njobs = 1;
     const uint32_t W = c->tune.window_bytes;
-    // windows of the range plus one split per logical-segment edge
-    P.desc.reserve((size_t)((std::min<uint64_t>(s1 * seg_num, N) - std::min<uint64_t>(s0 * seg_num, N)) /
-                            (30ull * W)) + 2 * (size_t)(s1 - s0) + 2);
-    for (uint64_t s = s0; s < s1; s++) {
-        const uint64_t a = s * seg_num + 1;
-        const uint64_t b = std::min<uint64_t>((s + 1) * seg_num, N);
-        plan_range(c, P, a, b, 0, (uint32_t)s, W, 0);
+    // window counts per logical segment, then the segments planned in
+    // parallel into their places (a plain run to 1e14: 3,334 segments,
+    // 2.5e7 windows)
+    const size_t nseg = (size_t)(s1 - s0);
+    std::vector<size_t> at(nseg + 1, 0);
+    auto range_of = [&](size_t i, uint64_t& a, uint64_t& b) {
+        const uint64_t s = s0 + i;
+        a = s * seg_num + 1;
+        b = std::min<uint64_t>((s + 1) * seg_num, N);
+    };
+    for (size_t i = 0; i < nseg; i++) {
+        uint64_t a, b;
+        range_of(i, a, b);
+        at[i + 1] = at[i] + plan_count(a, b, W, 0);
     }
+    P.desc.resize(at[nseg]);
+    const size_t per_seg = std::max<size_t>(1, at[nseg] / std::max<size_t>(nseg, 1));
+    parallel_for(nseg, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; i++) {
+            uint64_t a, b;
+            range_of(i, a, b);
+            plan_fill(c, P.desc.data() + at[i], a, b, 0, (uint32_t)(s0 + i), W, 0);
+        }
+    }, std::max<size_t>(1, (size_t)(1u << 16) / per_seg));
     return P;
 }
 
@@ -2096,7 +2222,7 @@ struct ServeReq {
     double t_submit = 0, t_done = 0;
     uint64_t hbm = 0;
     // interval requests
-    std::vector<SegDesc> win;  // planned windows, job = slot
+    DescVec win;  // planned windows, job = slot
     size_t next = 0;
     int slot = -1;
     JobHost J;
