@@ -227,7 +227,7 @@ struct voss_ctx {
     std::string name;
     int num_cus = 0;
     double total_gb = 0;
-    voss_tuning tune{131072, 1, 768, 1u << 21, 2, 100, {0, 0}};
+    voss_tuning tune{131072, 1, 768, 1u << 20, 2, 100, {0, 0}};
 
     std::vector<uint32_t> primes;
     std::vector<double> inv_prefix;  // inv_prefix[i] = sum of 1/p over primes[0..i)
@@ -1567,8 +1567,8 @@ int voss_autotune(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
     const std::vector<std::pair<uint32_t, uint32_t>> shapes = {
         {131072, 1}, {65536, 2}, {65536, 1}, {32768, 4}};
     const std::vector<uint32_t> tiers = {512, 768, 1024, 2048};
-    const std::vector<uint32_t> blimits = {1u << 17, 1u << 19, 1u << 21, 1u << 23, 1u << 25,
-                                           0xFFFFFFFFu};
+    const std::vector<uint32_t> blimits = {1u << 17, 1u << 18, 1u << 19, 1u << 20, 1u << 21,
+                                           1u << 22, 1u << 23, 1u << 25, 0xFFFFFFFFu};
     const std::vector<uint32_t> bmins = {2, 4, 8};
     for (int pass = 0; pass < 2; pass++) {
         bool moved = false;
