@@ -160,7 +160,7 @@ def batch_leg(eng, W, with_plain=True, autotune=True, mr_count=10**7):
     if autotune:
         t = time.perf_counter()
         best, sec = eng.autotune(lo, hi, max_intervals=256, repeats=1)
-        out["autotune"] = {"tuning": best, "sample_intervals": 256, "sample_s": sec,
+        out["autotune"] = {"tuning": best, "sample_intervals": 256, "tuned_s": sec,
                            "tuner_s": time.perf_counter() - t}
     off = vossgpu.IntervalResult.large.offset  # compare all fields but the pointer
 

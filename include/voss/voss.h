@@ -138,9 +138,11 @@ int voss_get_tuning(voss_ctx* ctx, voss_tuning* t);
 /* Autotuner: coordinate descent over window_bytes, tier_a_limit,
  * bucket_limit and bucket_min_windows, timing voss_run_batch on the given
  * intervals (at most max_intervals of them, evenly spread) with `repeats`
- * timed runs per candidate after one warm-up.  Installs the fastest tuning
- * on the context and returns it in best (and its time in best_seconds).
- * Results never depend on the tuning, only the time does. */
+ * timed runs per candidate after one warm-up; when that sample is a part of
+ * the n intervals, the chosen bucket_limit and its halves/doubles are then
+ * timed on all n.  Installs the fastest tuning on the context and returns
+ * it in best (and in best_seconds its time on the last set it was timed
+ * on).  Results never depend on the tuning, only the time does. */
 int voss_autotune(voss_ctx* ctx, const uint64_t* lo, const uint64_t* hi, size_t n,
                   size_t max_intervals, int repeats, voss_tuning* best,
                   double* best_seconds);

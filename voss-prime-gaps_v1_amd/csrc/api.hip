@@ -1601,6 +1601,41 @@ int voss_autotune(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
         }
         if (!moved) break;
     }
+    // the bucket limit trades per-window prime setup against bucket
+    // entries, a balance that moves with the batch's size: when the sample
+    // was a part of the batch, the limit and its two neighbours are timed
+    // once more on all n intervals
+    if (m < n && cur.bucket_limit != 0xFFFFFFFFu) {
+        std::vector<voss_interval_result> full(n);
+        auto time_full = [&](const voss_tuning& t, double* sec) -> int {
+            int rc2 = voss_set_tuning(c, &t);
+            if (rc2) return rc2;
+            double bestt = 1e300;
+            for (int r = 0; r <= repeats; r++) {
+                for (auto& x : full) { x.large = nullptr; x.large_capacity = 0; }
+                const double t0 = now_s();
+                if ((rc2 = voss_run_batch(c, lo, hi, n, full.data(), nullptr))) return rc2;
+                if (r > 0) bestt = std::min(bestt, now_s() - t0);
+            }
+            *sec = bestt;
+            return VOSS_OK;
+        };
+        double full_s = 0;
+        if ((rc = time_full(cur, &full_s))) return rc;
+        const voss_tuning base = cur;
+        for (const uint32_t bl : {base.bucket_limit / 2, base.bucket_limit * 2}) {
+            if (bl < base.window_bytes || bl > (1u << 30)) continue;
+            voss_tuning t = base;
+            t.bucket_limit = bl;
+            double sec = 0;
+            if ((rc = time_full(t, &sec))) return rc;
+            if (sec < full_s * 0.98) {
+                cur = t;
+                full_s = sec;
+            }
+        }
+        cur_s = full_s;
+    }
     if ((rc = voss_set_tuning(c, &cur))) return rc;
     if (best) *best = cur;
     if (best_seconds) *best_seconds = cur_s;
