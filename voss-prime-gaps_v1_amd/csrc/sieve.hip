@@ -44,6 +44,11 @@ __device__ __forceinline__ uint32_t w30_of(uint32_t o) {
 }
 __device__ __forceinline__ uint32_t q6row_of(uint32_t o) { return (0xD4u >> o) & 1u; }
 
+// Bit 8j+7 set iff byte j of w is nonzero.
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
+    return (((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u;
+}
+
 // x mod p for any 64-bit x: m = floor(2^64 / p) (Barrett, one correction).
 __device__ __forceinline__ uint32_t mod_u64(uint64_t x, uint32_t p,
                                             uint64_t m) {
@@ -224,6 +229,9 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
     // q=30 rows interleaved: mq[8*(g/2) + row] for gaps g < 100; bin 0 takes
     // the walk's gap-0 self-pairs and bin 50 its gaps >= 100 (both dropped)
     __shared__ uint32_t mq[64 * 8];
+    // walk: how many nonzero wheel bytes of each value were seen (their
+    // internal prime pairs go into mq at the flush)
+    __shared__ uint32_t s_bytes[EMIT ? 1 : 256];
     __shared__ uint32_t s_pat[kPresieveWordsPadded];
     __shared__ uint32_t tfirst[BLOCK];
     __shared__ uint32_t tlast[BLOCK];
@@ -238,12 +246,32 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
 
     for (int i = tid; i < kHistBins; i += BLOCK) hist[i] = 0;
     for (int i = tid; i < 8 * 64; i += BLOCK) mq[i] = 0;
+    if constexpr (!EMIT)
+        for (int i = tid; i < 256; i += BLOCK) s_bytes[i] = 0;
     for (int i = tid; i < kPresieveWordsPadded; i += BLOCK) s_pat[i] = a.pat[i];
     for (int i = tid; i < kLgtCount; i += BLOCK) s_lgt[i] = a.lg_thresh[kLgtFirst + i];
     uint32_t cur_job = blockIdx.x < a.ndesc ? a.desc[blockIdx.x].job : 0;
 
     auto flush = [&](uint32_t job) {
         __syncthreads();
+        if constexpr (!EMIT) {
+            // the consecutive set bits o1 < o2 of a byte value are a prime
+            // pair with gap W30[o2] - W30[o1] and p_before in row o1
+            for (int v = tid; v < 256; v += BLOCK) {
+                const uint32_t c = s_bytes[v];
+                if (!c) continue;
+                s_bytes[v] = 0;
+                uint32_t x = (uint32_t)v, o1 = __builtin_ctz(x);
+                x &= x - 1u;
+                while (x) {
+                    const uint32_t o2 = __builtin_ctz(x);
+                    x &= x - 1u;
+                    atomicAdd(&mq[((w30_of(o2) - w30_of(o1)) >> 1) * 8u + o1], c);
+                    o1 = o2;
+                }
+            }
+            __syncthreads();
+        }
         JobAcc* acc = &a.acc[job];
         for (int i = tid; i < kHistBins; i += BLOCK) {
             uint32_t v = hist[i];
@@ -460,7 +488,9 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
         // of v[k] is bit k of the number of words with bit j set (a run has
         // at most 32 words, so six planes hold every count)
         uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+        uint32_t nzb = 0;  // nonzero bytes of the run
         auto add_word = [&](uint32_t w) {
+            if constexpr (!EMIT) nzb += (uint32_t)__popc(nonzero_bytes(w));
             uint32_t c = w, t;
             t = v0 & c; v0 ^= c; c = t;
             t = v1 & c; v1 ^= c; c = t;
@@ -518,9 +548,7 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             emit_at = a.emit_off[d] + woff + incl - cnt;
             __syncthreads();
         }
-        // walk: primes of the run in increasing order.  The
-        // first prime of the run pairs with itself (gap 0, into the unused
-        // bin 0), so the loop body has no first-prime case.
+        // walk: the primes of the run in increasing order
         uint32_t first = 0, prev = 0;
         {
             uint32_t i = i0;
@@ -533,43 +561,64 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                 first = prevoff + 1u;
                 prev = cnt ? 1u : 0u;
             }
-            char* const mqb = (char*)mq;
-            // one prime: the lowest set bit of w (cleared), in word base
-            auto visit = [&](uint32_t& w, uint32_t base) {
-                const uint32_t b = __builtin_ctz(w);
-                w &= w - 1u;
-                // 30*(b>>3) + W30[b&7]; v_perm picks byte (b&7) of W30 and
-                // selector bytes 0x0C give zeros above it
-                const uint32_t w30 = __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u,
-                                                           (b & 7u) | 0x0C0C0C00u);
-                const uint32_t off = base + 30u * (b >> 3) + w30;
-                if constexpr (EMIT) {
-                    a.emit_out[emit_at++] = base_num + off;
-                } else {
-                    const uint32_t g = off - prevoff;
-                    // byte address 4*(8*(g/2) + row) = 16*g + 4*row (g even):
-                    // equal gaps of different rows fall in different banks.
-                    // Gaps >= 100 add into the unused bin 50 instead of
-                    // taking a branch around the atomic, and go to pair_wide.
-                    const uint32_t gn = min(g, 100u);
-                    atomicAdd((uint32_t*)(mqb + rowb + 16u * gn), 1u);
-                    if (g >= 100u) pair_wide(pc, prevoff, g);
-                    rowb = (b & 7u) << 2;
-                }
-                prevoff = off;
-            };
-            // one iteration per prime, not per word: a wave runs as many
-            // iterations as its lane with the most primes (a per-word loop
-            // runs, per word, as many as the lane with the fullest word).
-            // cnt counts the run's set bits, so the refill never passes iend.
             uint32_t w = i < iend ? bm[i] : 0u, base = 120u * i;
-            for (uint32_t k = 0; k < cnt; k++) {
-                while (w == 0u) {
-                    i++;
-                    w = bm[i];
-                    base += 120u;
+            if constexpr (EMIT) {
+                // every prime of the run in order, one iteration per prime
+                // (cnt counts the run's set bits, so the refill never passes
+                // iend)
+                for (uint32_t k = 0; k < cnt; k++) {
+                    while (w == 0u) {
+                        i++;
+                        w = bm[i];
+                        base += 120u;
+                    }
+                    const uint32_t b = __builtin_ctz(w);
+                    w &= w - 1u;
+                    // 30*(b>>3) + W30[b&7]; v_perm picks byte (b&7) of W30
+                    // and selector bytes 0x0C give zeros above it
+                    prevoff = base + 30u * (b >> 3) +
+                              __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, (b & 7u) | 0x0C0C0C00u);
+                    a.emit_out[emit_at++] = base_num + prevoff;
                 }
-                visit(w, base);
+            } else {
+                // one iteration per nonzero byte (30 numbers): its first
+                // prime pairs with the previous byte's last one here; the
+                // pairs inside the byte depend on the byte value alone and
+                // are counted by value (s_bytes), expanded at the flush.
+                // Bytes with primes are ~0.6x the primes at 1e11, and a
+                // wave runs as many iterations as its fullest lane.  The
+                // first prime of the run pairs with itself (gap 0 into the
+                // dropped bin 0).  mq byte address 4*(8*(g/2) + row) =
+                // 16*g + 4*row (g even): equal gaps of different rows fall
+                // in different banks; gaps >= 100 add into the dropped bin
+                // 50 instead of taking a branch around the atomic, and go
+                // to pair_wide.
+                char* const mqb = (char*)mq;
+                uint32_t m = nonzero_bytes(w);
+                for (uint32_t k = 0; k < nzb; k++) {
+                    while (m == 0u) {
+                        i++;
+                        w = bm[i];
+                        m = nonzero_bytes(w);
+                        base += 120u;
+                    }
+                    const uint32_t sh = (uint32_t)__builtin_ctz(m) - 7u;  // 8 * byte index
+                    m &= m - 1u;
+                    const uint32_t v = (w >> sh) & 0xFFu;
+                    const uint32_t blo = (uint32_t)__builtin_ctz(v);
+                    const uint32_t bhi = 31u - (uint32_t)__builtin_clz(v);
+                    const uint32_t bb = base + (sh >> 3) * 30u;
+                    const uint32_t off_lo =
+                        bb + __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, blo | 0x0C0C0C00u);
+                    const uint32_t off_hi =
+                        bb + __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, bhi | 0x0C0C0C00u);
+                    const uint32_t g = off_lo - prevoff;
+                    atomicAdd((uint32_t*)(mqb + rowb + 16u * min(g, 100u)), 1u);
+                    if (g >= 100u) pair_wide(pc, prevoff, g);
+                    atomicAdd(&s_bytes[v], 1u);
+                    rowb = bhi << 2;
+                    prevoff = off_hi;
+                }
             }
             prev = prev ? prevoff + 1u : 0u;
         }
