@@ -77,3 +77,17 @@ def test_plain_1e14_matches_reference_outputs(engine):
     assert ch[1] == [str(N), str(r["c1"] + 1), str(r["c3"] + 1), str(r["c3"] - r["c1"])]
     hl = read_rows("hl_trend_1e14.csv")
     assert hl[-1] == [str(N), f"{gaps[6] / gaps[2]:.6f}"]
+
+
+def test_large_gap_top_k_equals_full_sort(engine):
+    """A caller that keeps K of many large gaps gets the selection path
+    (per-thread best K, then the best K of those); it must equal the first
+    K of the fully sorted list, every field."""
+    N = 10**12
+    print("[gpu] plain [1, 1e12]: top-32 selection vs full sort", flush=True)
+    full = engine.run_plain(N, SEG_NUM, large_cap=1 << 22)
+    assert 4 * 65536 < full["large_count"] <= 1 << 22
+    assert len(full["large"]) == full["large_count"]
+    top = engine.run_plain(N, SEG_NUM, large_cap=32)
+    assert top["large_count"] == full["large_count"]
+    assert top["large"].tobytes() == full["large"][:32].tobytes()
