@@ -17,10 +17,12 @@
 //            Waves pull work from an LDS counter, smallest primes first,
 //            so the phase is balanced without a static split;
 //   phase 2  each thread walks its contiguous run of words in increasing
-//            order: popcounts give the per-residue and mod-4 counts, the
-//            set bits give consecutive primes, whose gaps go to LDS
-//            histograms; gaps that cross thread runs are joined after a
-//            barrier; gaps that cross windows are joined by stitch_kernel.
+//            order: popcounts give the per-residue and mod-4 counts; per
+//            nonzero wheel byte, its first prime pairs with the previous
+//            byte's last one (gap into the LDS histograms) and the pairs
+//            inside the byte are counted by byte value; gaps that cross
+//            thread runs are joined after a barrier; gaps that cross
+//            windows are joined by stitch_kernel.
 // The bitmap never leaves LDS.  HBM traffic per window is the descriptor,
 // one 24-byte record, and the (rare) large-gap records; histograms are
 // merged into HBM once per workgroup per job.
