@@ -297,7 +297,7 @@ def serve_leg(eng, W, n_req=300, n_cal=80):
 
     # continuous batching
     ids = [None] * n_req
-    with eng.serve(max_active=64, windows_per_step=2048, large_capacity=64) as srv:
+    with eng.serve(max_active=64, windows_per_step=4096, large_capacity=64) as srv:
         t0 = time.perf_counter()
 
         def submitter():
