@@ -472,7 +472,14 @@ def main():
                      "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / LDS_PEAK_GBS, "traffic": SIEVE_1E11_HBM_BYTES,
                      "algorithmic_bytes": lds_bytes, "m_eff": m_eff,
-                     "avg_kernel_ms": statistics.mean(sieve_ms)},
+                     "avg_kernel_ms": statistics.mean(sieve_ms),
+                     # the same work as dword read-modify-writes (M_eff per
+                     # launch) against the measured rate of random-address
+                     # ds_and_b32 on all CUs (lds_probe), the op the marks are
+                     "marks_per_s": m_eff / avg_sieve_s,
+                     "frac_of_measured_ds_and_rate": (
+                         m_eff / avg_sieve_s / (probe["ds_and_random_gops"] * 1e9)
+                         if probe.get("ds_and_random_gops") else None)},
         "device_ms_per_step": statistics.mean(dev_ms),
         "step_ms": [round(1e3 * x, 3) for x in step_s],
         "lds_probe": probe,
