@@ -514,7 +514,14 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                 add_word(w);
             }
         } else {
-            for (uint32_t i = i0; i < iend; i++) add_word(bm[i]);
+            // the order does not matter here: each lane starts at its own
+            // rotation of the run, so the 64 lanes of a wave (runs 32 words
+            // apart) read different banks instead of one
+            uint32_t i = i0 + (uint32_t)tid % per;
+            for (uint32_t q = 0; q < per; q++) {
+                add_word(bm[i]);
+                if (++i == iend) i = i0;
+            }
         }
         // counts from the planes: all primes, p % 4 == 1 (bytes 0 and 2 of a
         // word have even k: residues {1,13,17,29} there, {7,11,19,23} at odd
