@@ -110,6 +110,30 @@ def seeded_intervals(n, seed, lo_min=10**9, lo_max=10**13, lmin=3, lmax=7):
     return out
 
 
+def test_large_batch_host_paths(engine, oracle):
+    """2,500 intervals: enough for the multi-threaded host paths (planning
+    by job ranges, per-job accumulation, per-interval results).  Every
+    record equals the per-interval form's (planned and finished one
+    interval at a time) byte for byte; a sample also equals the oracle."""
+    import ctypes
+    import vossgpu
+    iv = seeded_intervals(2500, 0x564F5353 + 21, lo_min=10**9, lo_max=10**14, lmin=3, lmax=6.3)
+    iv += [(0, 1000), (10**6, 10**6), (7, 8), (2, 3)]
+    lo = [a for a, _ in iv]
+    hi = [b for _, b in iv]
+    print(f"[gpu] batch of {len(iv)} (multi-threaded host paths)", flush=True)
+    rb, lb, _ = engine.run_batch(lo, hi, large_cap=32, raw=True)
+    rp, lp, _ = engine.run_batch_plain(lo, hi, large_cap=32, raw=True)
+    off = vossgpu.IntervalResult.large.offset
+    for k in range(len(iv)):
+        assert bytes(rb[k])[:off] == bytes(rp[k])[:off], (k, iv[k])
+        n = min(rb[k].large_count, 32)
+        assert vossgpu.large_array(rb[k].large, n).tobytes() == \
+            vossgpu.large_array(rp[k].large, n).tobytes(), (k, iv[k])
+    for k in list(range(0, len(iv), 97)) + list(range(len(iv) - 4, len(iv))):
+        assert_same(vossgpu.interval_to_dict(rb[k]), oracle.run_interval(*iv[k]), large=False)
+
+
 def test_batch_matches_single_and_oracle(engine, oracle):
     iv = seeded_intervals(48, 0x564F5353 + 1)
     iv += [(0, 1000), (10**6, 10**6), (7, 8)]
