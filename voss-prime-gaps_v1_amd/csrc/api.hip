@@ -7,7 +7,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -167,21 +169,112 @@ bool merit_ge10(uint64_t p_before, uint32_t g) {
     return (double)g / std::log((double)p_before) >= 10.0;
 }
 
-// f(begin, end) over [0, n) split across up to 16 host threads when n is
-// large enough to pay for starting them.
+// A fixed set of host worker threads, started on first use, for the host
+// steps of a run (planning, accumulation, merits): starting threads per
+// call cost more than the sub-millisecond steps of a small run.  One
+// caller at a time; a second concurrent caller, or a call from inside a
+// worker, runs its work inline.
+class HostPool {
+public:
+    static HostPool& get() {
+        static HostPool pool;
+        return pool;
+    }
+    size_t workers() const { return th_.size(); }
+    // job(t) for t in [0, n), the calling thread taking part; false if the
+    // pool is busy (the caller then does the work itself)
+    bool run(size_t n, const std::function<void(size_t)>& job) {
+        if (in_worker() || !busy_.try_lock()) return false;
+        {
+            std::lock_guard<std::mutex> g(m_);
+            job_ = &job;
+            n_ = n;
+            next_ = 0;
+            done_ = 0;
+            gen_++;
+        }
+        cv_.notify_all();
+        in_worker() = true;  // a nested call from one of our jobs runs inline
+        work();
+        in_worker() = false;
+        {
+            std::unique_lock<std::mutex> g(m_);
+            done_cv_.wait(g, [&] { return done_ == n_; });
+            job_ = nullptr;
+        }
+        busy_.unlock();
+        return true;
+    }
+
+private:
+    HostPool() {
+        const size_t hw = std::max<unsigned>(1u, std::thread::hardware_concurrency());
+        const size_t nw = std::min<size_t>(15, hw > 1 ? hw - 1 : 0);
+        for (size_t i = 0; i < nw; i++) th_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    static bool& in_worker() {
+        static thread_local bool flag = false;
+        return flag;
+    }
+    // takes job indices until none is left
+    void work() {
+        for (;;) {
+            size_t t;
+            const std::function<void(size_t)>* job;
+            {
+                std::lock_guard<std::mutex> g(m_);
+                if (!job_ || next_ >= n_) return;
+                t = next_++;
+                job = job_;
+            }
+            (*job)(t);
+            std::lock_guard<std::mutex> g(m_);
+            if (++done_ == n_) done_cv_.notify_all();
+        }
+    }
+    void loop() {
+        in_worker() = true;
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_, busy_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(size_t)>* job_ = nullptr;
+    size_t n_ = 0, next_ = 0, done_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// f(begin, end) over [0, n) split into up to 16 chunks of at least
+// kMinPerThread items, run on the host pool.
 template <typename F>
 void parallel_for(size_t n, F&& f, size_t kMinPerThread = 8192) {
-    size_t nt = std::min<size_t>(16, std::max<unsigned>(1u, std::thread::hardware_concurrency()));
+    size_t nt = std::min<size_t>(16, HostPool::get().workers() + 1);
     nt = std::min(nt, std::max<size_t>(1, n / kMinPerThread));
     if (nt <= 1) {
         f((size_t)0, n);
         return;
     }
-    std::vector<std::thread> th;
-    for (size_t t = 1; t < nt; t++)
-        th.emplace_back([&, t] { f(n * t / nt, n * (t + 1) / nt); });
-    f((size_t)0, n / nt);
-    for (auto& x : th) x.join();
+    const std::function<void(size_t)> job = [&](size_t t) { f(n * t / nt, n * (t + 1) / nt); };
+    if (!HostPool::get().run(nt, job))
+        for (size_t t = 0; t < nt; t++) job(t);
 }
 
 // An allocator whose resize() leaves new elements default-initialised
