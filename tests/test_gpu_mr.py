@@ -54,3 +54,23 @@ def test_mr_near_powers_of_two(engine, oracle):
                  if 0 < (1 << k) + d < 2**64})
     ns = np.array(ns, dtype=np.uint64)
     assert np.array_equal(engine.mr(ns), oracle.is_prime_batch(ns))
+
+
+def test_mr_every_odd_below_2_31_against_a_sieve(engine):
+    """Every n < 2^31 against a sieve of Eratosthenes: the range in which
+    the kernels' witnesses are reduced mod n (the largest is 1795265022)
+    and a witness that n divides is skipped."""
+    lim = 1 << 31
+    odd_prime = np.ones(lim // 2, dtype=np.bool_)  # index i <-> 2i + 1
+    odd_prime[0] = False
+    for i in range(1, (int(lim ** 0.5) + 1) // 2 + 1):
+        if odd_prime[i]:
+            p = 2 * i + 1
+            odd_prime[p * p // 2::p] = False
+    assert int(odd_prime.sum()) + 1 == 105097565  # pi(2^31)
+    assert [int(x) for x in engine.mr(np.arange(0, 4, dtype=np.uint64))] == [0, 0, 1, 1]
+    chunk = 1 << 26
+    for start in range(0, lim // 2, chunk):
+        ns = np.arange(start, start + chunk, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+        v = engine.mr(ns)
+        assert np.array_equal(v.astype(np.bool_), odd_prime[start:start + chunk]), start

@@ -7,11 +7,22 @@
 // `%` (mul_mod, :24); here every product is a 64-bit Montgomery product
 // (R = 2^64) built from 32-bit multiplies, so no division runs in the loop.
 //
-// Two stages keep the waves coherent: stage 1 does trial division and the
-// base-2 test for every candidate (most composites stop here) and appends
-// the survivors to a compact list with one atomic per wave; stage 2 runs
-// bases 3..37 on the survivors only, which are almost all primes and so
-// all take the same number of steps.
+// The witnesses after trial division are not the reference's: base 2 and
+// then J. Sinclair's six bases 325, 9375, 28178, 450775, 9780504,
+// 1795265022, a set that is deterministic for every n < 2^64 (checked
+// against the Feitsma-Galway list of all base-2 strong pseudoprimes below
+// 2^64), with a base that n divides skipped.  The 12 prime bases are
+// deterministic for every n < 2^64 as well (Sorenson and Webster: up to
+// 3.18e23), so both tests return exact primality and the verdicts are the
+// same; a prime takes 7 powerings instead of 12.  Below 2^31, where the
+// bases are reduced mod n and can vanish, the verdicts are checked against
+// a sieve for every odd n.
+//
+// Three stages keep the waves coherent, each on a compact list of what the
+// previous one left open: trial division for every candidate, the base-2
+// test for those no small base divides (most composites stop here), then
+// the six other bases on the survivors only, which are almost all primes
+// and so all take the same number of steps.
 #include <hip/hip_runtime.h>
 
 #include "montgomery.h"
@@ -72,7 +83,8 @@ __device__ __forceinline__ bool sprp_window(const Mont& M, uint64_t bm, uint64_t
     return sprp_tail(M, x, s);
 }
 
-__constant__ uint32_t c_bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+// the witnesses after base 2 (deterministic with it below 2^64)
+__constant__ uint32_t c_bases[6] = {325u, 9375u, 28178u, 450775u, 9780504u, 1795265022u};
 
 // Divisibility of a 64-bit n by an odd constant b without a division:
 // b | n iff n * b^-1 mod 2^64 <= (2^64 - 1) / b (the multiples of b are
@@ -105,17 +117,41 @@ __device__ __forceinline__ int trial(uint64_t n) {
 
 // Survivors are appended with one atomic per workgroup pass (wave counts by
 // ballot, their prefix in LDS): a single counter hit once per wave
-// serialises at the L2 and bounded this kernel.
-__global__ __launch_bounds__(1024) void mr_stage1(const uint64_t* __restrict__ cand,
-                                                  uint64_t count,
-                                                  uint8_t* __restrict__ verdict,
-                                                  uint32_t* __restrict__ surv,
-                                                  uint32_t* __restrict__ nsurv) {
+// serialises at the L2 and bounded the first stage.  Every thread of the
+// block calls this once per pass.
+__device__ __forceinline__ void append_block(bool more, uint32_t value, uint32_t* list,
+                                             uint32_t* counter, uint32_t* s_wave,
+                                             uint32_t* s_base) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwaves = (int)(blockDim.x >> 6);
+    const uint64_t mask = __ballot(more);
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < nwaves; w++) {
+            const uint32_t c = s_wave[w];
+            s_wave[w] = t;
+            t += c;
+        }
+        *s_base = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    if (more) list[*s_base + s_wave[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+    __syncthreads();  // s_wave and s_base are rewritten by the next pass
+}
+
+// Stage 1a: trial division for every candidate; the verdict of the ones it
+// decides is written, the others (index) go to list 1.  Candidates that a
+// small base divides are most of a random set, so the powering stages
+// below run on a compact list instead of waves with those lanes idle.
+__global__ __launch_bounds__(1024) void mr_trial(const uint64_t* __restrict__ cand,
+                                                 uint64_t count, uint8_t* __restrict__ verdict,
+                                                 uint32_t* __restrict__ list1,
+                                                 uint32_t* __restrict__ n1) {
     __shared__ uint32_t s_wave[16];
     __shared__ uint32_t s_base;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nwaves = (int)(blockDim.x >> 6);
     // the loop bound depends on blockIdx only, so every thread of the
     // block takes every pass and reaches every barrier
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < count;
@@ -123,34 +159,40 @@ __global__ __launch_bounds__(1024) void mr_stage1(const uint64_t* __restrict__ c
         const uint64_t i = base + threadIdx.x;
         bool more = false;
         if (i < count) {
-            const uint64_t n = cand[i];
-            int v = trial(n);
-            if (v == 2) {
-                const Mont M = mont_setup(n);
-                uint64_t d = n - 1;
-                const int s = __ffsll((long long)d) - 1;
-                d >>= s;
-                v = sprp2(M, d, s) ? 2 : 0;
-            }
+            const int v = trial(cand[i]);
             more = (v == 2);
             verdict[i] = (uint8_t)(v == 1);
         }
-        const uint64_t mask = __ballot(more);
-        if (lane == 0) s_wave[wave] = (uint32_t)__popcll(mask);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t t = 0;
-            for (int w = 0; w < nwaves; w++) {
-                const uint32_t c = s_wave[w];
-                s_wave[w] = t;
-                t += c;
-            }
-            s_base = t ? atomicAdd(nsurv, t) : 0u;
+        append_block(more, (uint32_t)i, list1, n1, s_wave, &s_base);
+    }
+}
+
+// Stage 1b: the base-2 strong test on list 1; the candidates that pass go to
+// list 2 (their verdict is already 0 from stage 1a and is set by stage 2).
+__global__ __launch_bounds__(1024) void mr_base2(const uint64_t* __restrict__ cand,
+                                                 const uint32_t* __restrict__ list1,
+                                                 const uint32_t* __restrict__ n1,
+                                                 uint32_t* __restrict__ list2,
+                                                 uint32_t* __restrict__ n2) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_base;
+    const uint32_t count = *n1;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // the loop bound is uniform over the block (blocks past the list exit)
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t k = base + threadIdx.x;
+        bool more = false;
+        uint32_t i = 0;
+        if (k < count) {
+            i = list1[k];
+            const uint64_t n = cand[i];
+            const Mont M = mont_setup(n);
+            uint64_t d = n - 1;
+            const int s = __ffsll((long long)d) - 1;
+            d >>= s;
+            more = sprp2(M, d, s);
         }
-        __syncthreads();
-        if (more)
-            surv[s_base + s_wave[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)i;
-        __syncthreads();  // s_wave and s_base are rewritten by the next pass
+        append_block(more, i, list2, n2, s_wave, &s_base);
     }
 }
 
@@ -183,13 +225,15 @@ __global__ __launch_bounds__(256) void mr_stage2(const uint64_t* __restrict__ ca
         ktop = min(ktop, 20);
         if (!active) continue;
         const Mont M = mont_setup(n);
-        uint64_t bm = add_mod(M.one, M.one, n);  // base 2
-        uint32_t b = 2;
+        // R^2 mod n: 2R by a doubling, then six squarings (2^k R -> 2^2k R)
+        uint64_t r2 = add_mod(M.one, M.one, n);
+#pragma unroll
+        for (int j = 0; j < 6; j++) r2 = mont_sqr(r2, M);
         bool prime = true;
-        for (int t = 1; t < 12 && prime; t++) {
-            const uint32_t nb = c_bases[t];
-            for (; b < nb; b++) bm = add_mod(bm, M.one, n);
-            prime = sprp_window(M, bm, d, s, ktop, tbl);
+        for (int t = 0; t < 6 && prime; t++) {
+            // a R mod n = REDC(a * R^2); it is 0 exactly when n divides a
+            const uint64_t bm = mont_mul(c_bases[t], r2, M);
+            if (bm != 0) prime = sprp_window(M, bm, d, s, ktop, tbl);
         }
         verdict[i] = prime ? 1 : 0;
     }
@@ -200,22 +244,27 @@ hipError_t launch_mr(const uint64_t* d_cand, uint64_t count, uint8_t* d_verdict,
                      hipStream_t s) {
     if (count == 0) return hipSuccess;
     // a fill kernel, not hipMemsetAsync, so the sequence can be captured
-    hipError_t e = launch_fill_u32(d_nsurv, 1, 0, s);
+    hipError_t e = launch_fill_u32(d_nsurv, 2, 0, s);
     if (e != hipSuccess) return e;
     // one thread per candidate (the kernels grid-stride only beyond 2^32
     // threads): a capped grid with a few candidates per thread leaves its
-    // last pass half empty; stage 2 blocks beyond the survivor count exit
-    // at once
+    // last pass half empty; blocks of the later stages beyond their list's
+    // length exit at once
     (void)num_cus;
+    uint32_t* list1 = d_surv;
+    uint32_t* list2 = d_surv + count;
     uint64_t blocks1 = (count + 1023) / 1024;
     if (blocks1 > (1u << 22)) blocks1 = 1u << 22;
-    hipLaunchKernelGGL(mr_stage1, dim3((uint32_t)blocks1), dim3(1024), 0, s, d_cand,
-                       count, d_verdict, d_surv, d_nsurv);
+    hipLaunchKernelGGL(mr_trial, dim3((uint32_t)blocks1), dim3(1024), 0, s, d_cand, count,
+                       d_verdict, list1, d_nsurv);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(mr_base2, dim3((uint32_t)blocks1), dim3(1024), 0, s, d_cand, list1,
+                       d_nsurv, list2, d_nsurv + 1);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     uint64_t blocks = (count + 255) / 256;
     if (blocks > (1u << 24)) blocks = 1u << 24;
-    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(mr_stage2, dim3((uint32_t)blocks), dim3(256), 0, s, d_cand,
-                       d_surv, d_nsurv, d_verdict);
+                       list2, d_nsurv + 1, d_verdict);
     return hipGetLastError();
 }
 
