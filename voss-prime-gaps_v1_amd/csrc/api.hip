@@ -2108,7 +2108,7 @@ int voss_batch_add_mr(voss_batch* b, const uint64_t* d_n, size_t count, uint8_t*
     int rc = set_device(b->c);
     if (rc || (rc = b->X->ensure_side())) return rc;
     std::unique_ptr<DevBuf<uint32_t>> scratch(new DevBuf<uint32_t>());
-    HIPCHK(scratch->reserve(2 * count + 2));  // two survivor lists, then their counters
+    HIPCHK(scratch->reserve(2 * count + 3));  // two survivor lists, then their counters
     b->X->mr.push_back({d_n, count, d_verdict, scratch->p, scratch->p + 2 * count});
     b->mr_scratch.push_back(std::move(scratch));
     b->X->drop_graph();  // the next run captures the sequence with this batch in it
@@ -2233,7 +2233,7 @@ int voss_mr_batch_device(voss_ctx* c, const uint64_t* d_n, size_t count, uint8_t
         return VOSS_OK;
     }
     HIPCHK(c->d_surv.reserve(2 * count));
-    HIPCHK(c->d_nsurv.reserve(2));
+    HIPCHK(c->d_nsurv.reserve(3));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(launch_mr(d_n, count, d_verdict, c->d_surv.p, c->d_nsurv.p, c->num_cus, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -2520,7 +2520,7 @@ void serve_loop(voss_server* s) {
             all_n.reserve(mr_n);
             for (auto& r : admitted_mr) all_n.insert(all_n.end(), r->cand.begin(), r->cand.end());
             if (s->d_mr_n.reserve(mr_n) || s->d_mr_v.reserve(mr_n) || s->d_surv.reserve(2 * mr_n) ||
-                s->d_nsurv.reserve(2))
+                s->d_nsurv.reserve(3))
                 rc = fail(VOSS_ENOMEM, "MR buffers");
             else if (hipMemcpyAsync(s->d_mr_n.p, all_n.data(), mr_n * 8, hipMemcpyHostToDevice,
                                     s->mr_stream) != hipSuccess ||

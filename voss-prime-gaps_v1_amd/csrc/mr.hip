@@ -119,9 +119,11 @@ __device__ __forceinline__ int trial(uint64_t n) {
 // ballot, their prefix in LDS): a single counter hit once per wave
 // serialises at the L2 and bounded the first stage.  Every thread of the
 // block calls this once per pass.
+// With `from_end` the list is filled downwards from list[cap - 1] instead.
 __device__ __forceinline__ void append_block(bool more, uint32_t value, uint32_t* list,
                                              uint32_t* counter, uint32_t* s_wave,
-                                             uint32_t* s_base) {
+                                             uint32_t* s_base, bool from_end = false,
+                                             uint32_t cap = 0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nwaves = (int)(blockDim.x >> 6);
     const uint64_t mask = __ballot(more);
@@ -137,7 +139,10 @@ __device__ __forceinline__ void append_block(bool more, uint32_t value, uint32_t
         *s_base = t ? atomicAdd(counter, t) : 0u;
     }
     __syncthreads();
-    if (more) list[*s_base + s_wave[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+    if (more) {
+        const uint32_t at = *s_base + s_wave[wave] + __popcll(mask & ((1ull << lane) - 1ull));
+        list[from_end ? cap - 1 - at : at] = value;
+    }
     __syncthreads();  // s_wave and s_base are rewritten by the next pass
 }
 
@@ -167,12 +172,18 @@ __global__ __launch_bounds__(1024) void mr_trial(const uint64_t* __restrict__ ca
     }
 }
 
+// The windowed powering of stage 2 runs as many base-8 digits as the
+// longest exponent of the wave, so its list is kept in two parts: d < 2^57
+// from the front, longer exponents from the back (the two never meet: both
+// together hold at most `cap` entries).
+constexpr int kLongExponentBits = 57;
+
 // Stage 1b: the base-2 strong test on list 1; the candidates that pass go to
 // list 2 (their verdict is already 0 from stage 1a and is set by stage 2).
 __global__ __launch_bounds__(1024) void mr_base2(const uint64_t* __restrict__ cand,
                                                  const uint32_t* __restrict__ list1,
                                                  const uint32_t* __restrict__ n1,
-                                                 uint32_t* __restrict__ list2,
+                                                 uint32_t* __restrict__ list2, uint32_t cap,
                                                  uint32_t* __restrict__ n2) {
     __shared__ uint32_t s_wave[16];
     __shared__ uint32_t s_base;
@@ -181,7 +192,7 @@ __global__ __launch_bounds__(1024) void mr_base2(const uint64_t* __restrict__ ca
     // the loop bound is uniform over the block (blocks past the list exit)
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t k = base + threadIdx.x;
-        bool more = false;
+        bool more = false, longer = false;
         uint32_t i = 0;
         if (k < count) {
             i = list1[k];
@@ -191,17 +202,22 @@ __global__ __launch_bounds__(1024) void mr_base2(const uint64_t* __restrict__ ca
             const int s = __ffsll((long long)d) - 1;
             d >>= s;
             more = sprp2(M, d, s);
+            longer = (d >> kLongExponentBits) != 0;
         }
-        append_block(more, i, list2, n2, s_wave, &s_base);
+        append_block(more && !longer, i, list2, n2, s_wave, &s_base);
+        append_block(more && longer, i, list2, n2 + 1, s_wave, &s_base, true, cap);
     }
 }
 
 __global__ __launch_bounds__(256) void mr_stage2(const uint64_t* __restrict__ cand,
                                                  const uint32_t* __restrict__ surv,
+                                                 uint32_t cap,
                                                  const uint32_t* __restrict__ nsurv,
                                                  uint8_t* __restrict__ verdict) {
     __shared__ uint64_t tbl[8 * 256];
-    const uint32_t ns = *nsurv;
+    // the front part of the list, then the back part read downwards
+    const uint32_t nfront = nsurv[0];
+    const uint32_t ns = nfront + nsurv[1];
     const uint32_t stride = gridDim.x * blockDim.x;
     // the loop bound is uniform over the block, so the whole wave is present
     // for the digit-count reduction below
@@ -212,7 +228,7 @@ __global__ __launch_bounds__(256) void mr_stage2(const uint64_t* __restrict__ ca
         int s = 1;
         uint32_t i = 0;
         if (active) {
-            i = surv[k];
+            i = surv[k < nfront ? k : cap - 1 - (k - nfront)];
             n = cand[i];
             d = n - 1;
             s = __ffsll((long long)d) - 1;
@@ -244,7 +260,7 @@ hipError_t launch_mr(const uint64_t* d_cand, uint64_t count, uint8_t* d_verdict,
                      hipStream_t s) {
     if (count == 0) return hipSuccess;
     // a fill kernel, not hipMemsetAsync, so the sequence can be captured
-    hipError_t e = launch_fill_u32(d_nsurv, 2, 0, s);
+    hipError_t e = launch_fill_u32(d_nsurv, 3, 0, s);
     if (e != hipSuccess) return e;
     // one thread per candidate (the kernels grid-stride only beyond 2^32
     // threads): a capped grid with a few candidates per thread leaves its
@@ -259,12 +275,12 @@ hipError_t launch_mr(const uint64_t* d_cand, uint64_t count, uint8_t* d_verdict,
                        d_verdict, list1, d_nsurv);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(mr_base2, dim3((uint32_t)blocks1), dim3(1024), 0, s, d_cand, list1,
-                       d_nsurv, list2, d_nsurv + 1);
+                       d_nsurv, list2, (uint32_t)count, d_nsurv + 1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     uint64_t blocks = (count + 255) / 256;
     if (blocks > (1u << 24)) blocks = 1u << 24;
     hipLaunchKernelGGL(mr_stage2, dim3((uint32_t)blocks), dim3(256), 0, s, d_cand,
-                       list2, d_nsurv + 1, d_verdict);
+                       list2, (uint32_t)count, d_nsurv + 1, d_verdict);
     return hipGetLastError();
 }
 

@@ -6,7 +6,7 @@
 namespace voss {
 
 // verdict[i] = 1 iff cand[i] is prime.  d_surv must hold 2 * count entries
-// (two survivor lists); d_nsurv holds their two counters.  count must be
+// (two survivor lists); d_nsurv holds their three counters.  count must be
 // below 2^32.
 hipError_t launch_mr(const uint64_t* d_cand, uint64_t count, uint8_t* d_verdict,
                      uint32_t* d_surv, uint32_t* d_nsurv, int num_cus,
