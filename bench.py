@@ -485,16 +485,17 @@ def main():
     mr_achieved = work / (statistics.median(mr_ms) / 1e3)
     mr_executed = executed / (statistics.median(mr_ms) / 1e3)
     if probe and probe.get("montmul_per_s"):
-        # achieved counts the reference algorithm's products (12 witnesses);
-        # the kernels run fewer (7 witnesses, windowed powering), and
-        # frac_executed is what they run against the measured product rate
-        mr["roofline"] = {"bound": "valu", "achieved": mr_achieved,
+        # achieved counts the products the kernels run (7 witnesses,
+        # windowed powering); the reference algorithm's products (12
+        # witnesses, binary powering) over the same time are reported beside
+        # it and exceed the measured rate, since the kernels run fewer
+        mr["roofline"] = {"bound": "valu", "achieved": mr_executed,
                           "peak": probe["montmul_per_s"], "unit": "mulmod/s",
-                          "frac": mr_achieved / probe["montmul_per_s"],
-                          "mulmods_per_test": work / len(cand),
-                          "executed_mulmods_per_test": executed / len(cand),
-                          "executed_per_s": mr_executed,
-                          "frac_executed": mr_executed / probe["montmul_per_s"],
+                          "frac": mr_executed / probe["montmul_per_s"],
+                          "mulmods_per_test": executed / len(cand),
+                          "reference_mulmods_per_test": work / len(cand),
+                          "reference_work_per_s": mr_achieved,
+                          "frac_reference_work": mr_achieved / probe["montmul_per_s"],
                           "sample": f"exact count on the first {n_sample} candidates"}
     if not args.no_cpu:
         from oracle import oracle as O
