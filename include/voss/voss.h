@@ -126,7 +126,11 @@ typedef struct {
 /* ---------------------------------------------------------------- context */
 
 /* Replaces the reference's per-run setup in main(): cudaGetDeviceProperties,
- * cudaStreamCreate and the cudaMalloc block (src/voss_v7_live.cu:353-401). */
+ * cudaStreamCreate and the cudaMalloc block (src/voss_v7_live.cu:353-401).
+ * A context is used by one host thread at a time, and one process drives
+ * one context at a time: calls on two contexts from two threads at once
+ * are not supported (such a run stopped making progress on an MI355X, cause
+ * not found).  The serve API is the entry point for concurrent clients. */
 int voss_create(int device, voss_ctx** out);
 void voss_destroy(voss_ctx* ctx);
 const char* voss_last_error(void);
