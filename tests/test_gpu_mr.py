@@ -74,3 +74,27 @@ def test_mr_every_odd_below_2_31_against_a_sieve(engine):
         ns = np.arange(start, start + chunk, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
         v = engine.mr(ns)
         assert np.array_equal(v.astype(np.bool_), odd_prime[start:start + chunk]), start
+
+
+def test_mr_base2_pseudoprimes_above_2_31(engine, oracle):
+    """Base-2 strong pseudoprimes above 2^31 (semiprimes p(2p - 1) and
+    Chernick Carmichael numbers, built from their factors): the kernels pass
+    them through the base-2 stage and the further witnesses must reject
+    every one, as the oracle's do."""
+    from pseudoprimes import chernick_carmichaels, semiprime_pseudoprimes
+    semi = semiprime_pseudoprimes(oracle.is_prime_batch, 0x564F5353 + 41)
+    carm = chernick_carmichaels(oracle.is_prime_batch)
+    ns = np.array(semi + [n for n, _ in carm], dtype=np.uint64)
+    print(f"[gpu] {len(ns)} base-2 pseudoprimes", flush=True)
+    assert len(ns) > 3000
+    # interleaved with primes so the waves also hold lanes that run every witness
+    rng = np.random.default_rng(0x564F5353 + 42)
+    odd = rng.integers(2**31, 2**63, size=40 * len(ns), dtype=np.uint64) | np.uint64(1)
+    primes = odd[oracle.is_prime_batch(odd).astype(bool)]
+    assert len(primes) >= len(ns)
+    mix = np.empty(2 * len(ns), dtype=np.uint64)
+    mix[0::2] = ns
+    mix[1::2] = primes[:len(ns)]
+    v = engine.mr(mix)
+    assert not v[0::2].any()
+    assert np.array_equal(v, oracle.is_prime_batch(mix))

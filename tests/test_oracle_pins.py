@@ -271,3 +271,15 @@ def test_verify_gap_semantics(oracle):
     assert oracle.verify_gap(25056082543, 456)   # maximal gap below 1e11
     assert not oracle.verify_gap(25056082543, 454)
     assert oracle.verify_gap(7, 0)           # gap < 2: only the ends
+
+
+def test_mr_base2_pseudoprimes_above_2_31(oracle):
+    """Composites built from their factors that pass the base-2 strong test:
+    the oracle's later witnesses reject every one (and base 2 alone would
+    not: each is checked to pass it)."""
+    from pseudoprimes import chernick_carmichaels, semiprime_pseudoprimes
+    semi = semiprime_pseudoprimes(oracle.is_prime_batch, 0x564F5353 + 40, starts=2)
+    carm = chernick_carmichaels(oracle.is_prime_batch)
+    assert len(semi) > 500 and len(carm) > 1000 and sum(s for _, s in carm) > 100
+    ns = np.array(semi + [n for n, _ in carm], dtype=np.uint64)
+    assert not oracle.is_prime_batch(ns).any()
