@@ -96,10 +96,6 @@ def measured_lds_peaks(copy_bytes=0):
 
 
 MR_BASES = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37)
-# the device's witnesses after trial division by MR_BASES (csrc/mr.hip)
-MR_DEVICE_BASES = (2, 325, 9375, 28178, 450775, 9780504, 1795265022)
-
-
 def _tail_squarings(x, n, r):
     """Squarings the witness loop runs after x = b^d mod n, and its verdict."""
     if x == 1 or x == n - 1:
@@ -116,11 +112,11 @@ def _tail_squarings(x, n, r):
 def mr_mulmods_device(n):
     """Modular multiplications the device kernels run for n (csrc/mr.hip):
     base 2 by binary powering with doublings for the set bits (bitlen(d) - 1
-    squarings), then for each further witness one product for a*R, six
-    for the table of powers and four per base-8 digit below the top one
-    (the lane's own digit count; a wave runs its largest), plus six
-    squarings for R^2 mod n; the witness-loop squarings that run are
-    counted in both stages."""
+    squarings), then bases 3..37 by fixed 3-bit windows: six products for
+    the table of powers and four per base-8 digit below the top one (the
+    lane's own digit count; a wave runs its largest); the witness-loop
+    squarings that run are counted for every base.  Trial division and the
+    modular additions that step from one base to the next cost 0."""
     if n < 2:
         return 0
     for b in MR_BASES:
@@ -135,12 +131,8 @@ def mr_mulmods_device(n):
     work += t
     if not ok:
         return work
-    work += 6
     digits = (d.bit_length() - 1) // 3
-    for b in MR_DEVICE_BASES[1:]:
-        work += 1
-        if b % n == 0:
-            continue
+    for b in MR_BASES[1:]:
         t, ok = _tail_squarings(pow(b, d, n), n, r)
         work += 6 + 4 * digits + t
         if not ok:
@@ -485,10 +477,9 @@ def main():
     mr_achieved = work / (statistics.median(mr_ms) / 1e3)
     mr_executed = executed / (statistics.median(mr_ms) / 1e3)
     if probe and probe.get("montmul_per_s"):
-        # achieved counts the products the kernels run (7 witnesses,
-        # windowed powering); the reference algorithm's products (12
-        # witnesses, binary powering) over the same time are reported beside
-        # it and exceed the measured rate, since the kernels run fewer
+        # achieved counts the products the kernels run (windowed powering
+        # for bases 3..37); the reference algorithm's products (binary
+        # powering for every base) over the same time are reported beside it
         mr["roofline"] = {"bound": "valu", "achieved": mr_executed,
                           "peak": probe["montmul_per_s"], "unit": "mulmod/s",
                           "frac": mr_executed / probe["montmul_per_s"],

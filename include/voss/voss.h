@@ -228,10 +228,8 @@ void voss_batch_free(voss_batch* batch);
 
 /* ----------------------------------------------------------- Miller-Rabin */
 
-/* Deterministic Miller-Rabin on the GPU (64-bit Montgomery form): replaces
- * is_prime_u64 / mod_pow / mul_mod (src/verify_cpp.cpp:24-57) with the same
- * verdict for every 64-bit n (its trial division by the 12 prime bases,
- * then 7 witnesses that, like its 12, are deterministic below 2^64).
+/* Deterministic 12-base Miller-Rabin on the GPU (64-bit Montgomery form):
+ * replaces is_prime_u64 / mod_pow / mul_mod (src/verify_cpp.cpp:24-57).
  * verdict[i] = 1 iff n[i] is prime.  Host buffers. */
 int voss_mr_batch(voss_ctx* ctx, const uint64_t* n, size_t count,
                   uint8_t* verdict);

@@ -7,22 +7,11 @@
 // `%` (mul_mod, :24); here every product is a 64-bit Montgomery product
 // (R = 2^64) built from 32-bit multiplies, so no division runs in the loop.
 //
-// The witnesses after trial division are not the reference's: base 2 and
-// then J. Sinclair's six bases 325, 9375, 28178, 450775, 9780504,
-// 1795265022, a set that is deterministic for every n < 2^64 (checked
-// against the Feitsma-Galway list of all base-2 strong pseudoprimes below
-// 2^64), with a base that n divides skipped.  The 12 prime bases are
-// deterministic for every n < 2^64 as well (Sorenson and Webster: up to
-// 3.18e23), so both tests return exact primality and the verdicts are the
-// same; a prime takes 7 powerings instead of 12.  Below 2^31, where the
-// bases are reduced mod n and can vanish, the verdicts are checked against
-// a sieve for every odd n.
-//
 // Three stages keep the waves coherent, each on a compact list of what the
-// previous one left open: trial division for every candidate, the base-2
-// test for those no small base divides (most composites stop here), then
-// the six other bases on the survivors only, which are almost all primes
-// and so all take the same number of steps.
+// previous one left open: trial division by the 12 bases for every
+// candidate, the base-2 test for those no base divides (most composites
+// stop here), then bases 3..37 on the survivors only, which are almost all
+// primes and so all take the same number of steps.
 #include <hip/hip_runtime.h>
 
 #include "montgomery.h"
@@ -83,8 +72,8 @@ __device__ __forceinline__ bool sprp_window(const Mont& M, uint64_t bm, uint64_t
     return sprp_tail(M, x, s);
 }
 
-// the witnesses after base 2 (deterministic with it below 2^64)
-__constant__ uint32_t c_bases[6] = {325u, 9375u, 28178u, 450775u, 9780504u, 1795265022u};
+// the bases after 2
+__constant__ uint32_t c_bases[11] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
 
 // Divisibility of a 64-bit n by an odd constant b without a division:
 // b | n iff n * b^-1 mod 2^64 <= (2^64 - 1) / b (the multiples of b are
@@ -241,15 +230,14 @@ __global__ __launch_bounds__(256) void mr_stage2(const uint64_t* __restrict__ ca
         ktop = min(ktop, 20);
         if (!active) continue;
         const Mont M = mont_setup(n);
-        // R^2 mod n: 2R by a doubling, then six squarings (2^k R -> 2^2k R)
-        uint64_t r2 = add_mod(M.one, M.one, n);
-#pragma unroll
-        for (int j = 0; j < 6; j++) r2 = mont_sqr(r2, M);
+        // each base in Montgomery form by modular additions from the last
+        // (n > 37 here, so no base is 0 mod n)
+        uint64_t bm = add_mod(M.one, M.one, n);  // base 2
+        uint32_t b = 2;
         bool prime = true;
-        for (int t = 0; t < 6 && prime; t++) {
-            // a R mod n = REDC(a * R^2); it is 0 exactly when n divides a
-            const uint64_t bm = mont_mul(c_bases[t], r2, M);
-            if (bm != 0) prime = sprp_window(M, bm, d, s, ktop, tbl);
+        for (int t = 0; t < 11 && prime; t++) {
+            for (const uint32_t nb = c_bases[t]; b < nb; b++) bm = add_mod(bm, M.one, n);
+            prime = sprp_window(M, bm, d, s, ktop, tbl);
         }
         verdict[i] = prime ? 1 : 0;
     }
