@@ -1778,14 +1778,38 @@ Plan plain_plan(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t s0, uint64_t
         at[i + 1] = at[i] + plan_count(a, b, W, 0);
     }
     P.desc.resize(at[nseg]);
-    const size_t per_seg = std::max<size_t>(1, at[nseg] / std::max<size_t>(nseg, 1));
-    parallel_for(nseg, [&](size_t i0, size_t i1) {
-        for (size_t i = i0; i < i1; i++) {
-            uint64_t a, b;
-            range_of(i, a, b);
-            plan_fill(c, P.desc.data() + at[i], a, b, 0, (uint32_t)(s0 + i), W, 0);
+    // pieces of at most kPiece windows (a segment's windows k = ka..kb of
+    // the grid 30*W*k; a piece clips [a, b] to its windows, which plan_fill
+    // then produces exactly as for the whole segment), filled in parallel
+    constexpr size_t kPiece = 2048;
+    struct Piece {
+        uint64_t a, b;
+        size_t at;
+        uint32_t seg;
+    };
+    std::vector<Piece> pieces;
+    pieces.reserve(at[nseg] / kPiece + nseg);
+    for (size_t i = 0; i < nseg; i++) {
+        uint64_t a, b;
+        range_of(i, a, b);
+        for (size_t k = 0; at[i] + k < at[i + 1]; k += kPiece) {
+            uint64_t pa = a, pb = b;
+            if (k) {  // the piece starts at its first window
+                const uint64_t ka = (std::max<uint64_t>(a, 7) / 30) / W;
+                pa = std::max<uint64_t>(a, 30 * (ka + k) * W);
+            }
+            if (at[i] + k + kPiece < at[i + 1]) {  // and ends at its last
+                const uint64_t ka = (std::max<uint64_t>(a, 7) / 30) / W;
+                pb = std::min<uint64_t>(b, 30 * (ka + k + kPiece) * W - 1);
+            }
+            pieces.push_back({pa, pb, at[i] + k, (uint32_t)(s0 + i)});
         }
-    }, std::max<size_t>(1, (size_t)(1u << 16) / per_seg));
+    }
+    parallel_for(pieces.size(), [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; i++)
+            plan_fill(c, P.desc.data() + pieces[i].at, pieces[i].a, pieces[i].b, 0, pieces[i].seg,
+                      W, 0);
+    }, 1);
     return P;
 }
 
