@@ -692,42 +692,77 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
 // reference, src/voss_v7_live.cu:551-562, generalised to every window
 // edge).  Such a pair counts in-segment only when both windows carry the
 // same logical segment.
+// When every window of the workgroup belongs to one job (always in a plain
+// run), its pair counts are summed in LDS and flushed with one global
+// atomic per nonzero bin: in a plain run every thread's pair lands on the
+// few hundred bins of one job, and global atomics on the same addresses
+// serialise.  The flag is uniform over the workgroup, so are its barriers.
 __global__ void stitch_kernel(SieveArgs a) {
-    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= a.ndesc) return;
-    const SegRec rd = a.rec[d];
-    if (rd.count == 0) return;
-    const uint32_t job = a.desc[d].job;
-    int e = (int)d - 1;
-    for (; e >= 0 && a.desc[e].job == job; e--)
-        if (a.rec[e].count) break;
-    if (e < 0 || a.desc[e].job != job) return;
-    const unsigned long long p = a.rec[e].last;
-    const unsigned long long g64 = rd.first - p;
-    JobAcc* acc = &a.acc[job];
-    if (g64 >= 2ull * kHistBins) {
-        atomicOr(&acc->error, 1ull);
-        return;
+    __shared__ uint32_t s_hist[kHistBins];
+    __shared__ uint32_t s_mq[kModqRows * kModqBins];
+    const uint32_t dlo = blockIdx.x * blockDim.x;  // < ndesc: one block per 256
+    const uint32_t dhi = min(dlo + blockDim.x, a.ndesc) - 1u;
+    const uint32_t job0 = a.desc[dlo].job;
+    const bool local = a.desc[dhi].job == job0;
+    if (local) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kHistBins; i += blockDim.x) s_hist[i] = 0;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(kModqRows * kModqBins); i += blockDim.x)
+            s_mq[i] = 0;
+        __syncthreads();
     }
-    const uint32_t g = (uint32_t)g64;
-    atomicAdd(&acc->hist[g >> 1], 1ull);
-    if (a.desc[e].lseg != a.desc[d].lseg) return;
-    if (g < 100u) {
-        const uint32_t o = (uint32_t)c_residx[p % 30ull];
-        atomicAdd(&acc->modq[q6row_of(o) * kModqBins + (g >> 1)], 1ull);
-        atomicAdd(&acc->modq[(2u + o) * kModqBins + (g >> 1)], 1ull);
-    } else if (g > 100u) {
-        if (g >= (uint32_t)kLargeAlways || p <= a.lg_thresh[g >> 1]) {
-            unsigned long long slot = atomicAdd(a.large_n, 1ull);
-            atomicAdd(&acc->large_count, 1ull);
-            if (slot < a.large_cap) {
-                LargeRaw r;
-                r.p_before = p;
-                r.gap = g;
-                r.job = job;
-                a.large[slot] = r;
+    const uint32_t d = dlo + threadIdx.x;
+    do {  // one pass; break = this window adds nothing
+        if (d >= a.ndesc) break;
+        const SegRec rd = a.rec[d];
+        if (rd.count == 0) break;
+        const uint32_t job = a.desc[d].job;
+        int e = (int)d - 1;
+        for (; e >= 0 && a.desc[e].job == job; e--)
+            if (a.rec[e].count) break;
+        if (e < 0 || a.desc[e].job != job) break;
+        const unsigned long long p = a.rec[e].last;
+        const unsigned long long g64 = rd.first - p;
+        JobAcc* acc = &a.acc[job];
+        if (g64 >= 2ull * kHistBins) {
+            atomicOr(&acc->error, 1ull);
+            break;
+        }
+        const uint32_t g = (uint32_t)g64;
+        if (local) atomicAdd(&s_hist[g >> 1], 1u);
+        else atomicAdd(&acc->hist[g >> 1], 1ull);
+        if (a.desc[e].lseg != a.desc[d].lseg) break;
+        if (g < 100u) {
+            const uint32_t o = (uint32_t)c_residx[p % 30ull];
+            const uint32_t i6 = q6row_of(o) * kModqBins + (g >> 1);
+            const uint32_t i30 = (2u + o) * kModqBins + (g >> 1);
+            if (local) {
+                atomicAdd(&s_mq[i6], 1u);
+                atomicAdd(&s_mq[i30], 1u);
+            } else {
+                atomicAdd(&acc->modq[i6], 1ull);
+                atomicAdd(&acc->modq[i30], 1ull);
+            }
+        } else if (g > 100u) {
+            if (g >= (uint32_t)kLargeAlways || p <= a.lg_thresh[g >> 1]) {
+                unsigned long long slot = atomicAdd(a.large_n, 1ull);
+                atomicAdd(&acc->large_count, 1ull);
+                if (slot < a.large_cap) {
+                    LargeRaw r;
+                    r.p_before = p;
+                    r.gap = g;
+                    r.job = job;
+                    a.large[slot] = r;
+                }
             }
         }
+    } while (false);
+    if (local) {
+        __syncthreads();
+        JobAcc* acc = &a.acc[job0];
+        for (uint32_t i = threadIdx.x; i < (uint32_t)kHistBins; i += blockDim.x)
+            if (s_hist[i]) atomicAdd(&acc->hist[i], (unsigned long long)s_hist[i]);
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(kModqRows * kModqBins); i += blockDim.x)
+            if (s_mq[i]) atomicAdd(&acc->modq[i], (unsigned long long)s_mq[i]);
     }
 }
 
