@@ -267,6 +267,41 @@ def large_endpoints(res):
     return np.array(ends, dtype=np.uint64)
 
 
+def verify_leg(eng, large, no_cpu):
+    """verify_gap (src/verify_cpp.cpp:59-93) on the GPU over the 1e11 run's
+    whole large-gap list -- what run_voss.py hands to verify_cpp after the
+    run -- in gaps/s (host buffers, both PCIe copies inside), the median of
+    5 after a warm-up.  Beside it on the same list: the oracle's verify_gap
+    restatement on the host cores.  (The reference's verify_cpp process is
+    not timed: its progress thread sleeps 30 s before its first check and is
+    joined before the end, src/verify_cpp.cpp:196-219, so a list that
+    verifies in less takes 30 s whatever its size.)"""
+    pa = np.ascontiguousarray(large["position"], dtype=np.uint64)
+    g = np.ascontiguousarray(large["gap"], dtype=np.int32)
+    eng.verify_gaps(pa[:1000], g[:1000])
+    ts = []
+    for _ in range(5):
+        t = time.perf_counter()
+        ok = eng.verify_gaps(pa, g)
+        ts.append(time.perf_counter() - t)
+    assert ok.all()
+    sec = statistics.median(ts)
+    out = {"gaps": int(len(pa)), "seconds": sec, "gaps_per_s": len(pa) / sec,
+           "list": "large gaps of plain [1, 1e11] (gap >= 500, or > 100 with merit >= 10)"}
+    if no_cpu:
+        return out
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    ref = O.verify_gaps(pa, g, threads=threads)
+    dt = time.perf_counter() - t
+    assert np.array_equal(ref, ok)
+    out["cpu_baseline"] = {"value": len(pa) / dt, "unit": "gaps/s", "cores": threads,
+                           "kind": "port", "sample": "the whole list, verify_gap restated "
+                           "with __uint128_t (oracle, OpenMP)"}
+    return out
+
+
 def batch_with_mr(eng, lo, hi, cand, rb, same, replay_s, out):
     """Prepared batch + device-resident MR batch in one graph: replay time,
     against the batch alone plus MR alone; verdicts against voss_mr_batch."""
@@ -399,8 +434,8 @@ def main():
                     help="skip the per-interval comparison form of the batch leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="run the batch leg with the default tuning")
-    ap.add_argument("--legs", default="interval,batch,serve",
-                    help="comma-separated legs after the headline: interval, batch, serve, "
+    ap.add_argument("--legs", default="interval,batch,serve,verify",
+                    help="comma-separated legs after the headline: interval, batch, serve, verify, "
                          "plain1e12, plain1e13, plain1e14 (not default: 0.2 s, 2.7 s, 35 s)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -442,6 +477,8 @@ def main():
                                              mr_count=args.mr_count)
     if "serve" in want:
         legs["serve"] = serve_leg(eng, W)
+    if "verify" in want:
+        legs["verify_1e11"] = verify_leg(eng, r["large"], args.no_cpu)
     # plain runs at the reference's larger published scales (not in the
     # default set: [1, 1e14] takes about 35 s); pi(N) from README.md:122-127
     for e, pi, ref_s in ((12, 37607912018, 68.5), (13, 346065536839, 954.0),

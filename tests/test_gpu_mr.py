@@ -41,6 +41,14 @@ def test_verify_gaps(engine, oracle):
     rows = [(p, g) for p, g, _ in r["large"]]
     rows += [(p, g - 2) for p, g in rows[:40]] + [(p + 2, g + 2) for p, g in rows[:40]]
     rows += [(11, 4), (13, 6), (7, 0), (15, 4), (9, 1), (25056082543, 456), (25056082543, 454)]
+    rows += [(3, 1), (5, 3), (7, 5), (7, 2), (5, 2), (29, 27), (37, 2)]
+    # claimed gaps beyond the per-thread interior check (65,536), both ends
+    # prime: they go through the interval-sieve path and fail (primes inside)
+    for pa in (1000000007, 10**12 + 39):
+        g = 65538
+        while not oracle.is_prime(pa - g):
+            g += 2
+        rows.append((pa, g))
     pa = np.array([p for p, _ in rows], dtype=np.uint64)
     g = np.array([x for _, x in rows], dtype=np.int32)
     assert np.array_equal(engine.verify_gaps(pa, g), oracle.verify_gaps(pa, g))

@@ -354,6 +354,9 @@ struct voss_ctx {
     DevBuf<uint8_t> d_mr_v;
     DevBuf<uint32_t> d_surv;
     DevBuf<uint32_t> d_nsurv;
+    DevBuf<uint64_t> d_vf_pa;   // verify_gaps: p_after, gap, ok
+    DevBuf<int32_t> d_vf_gap;
+    DevBuf<uint8_t> d_vf_ok;
 };
 
 namespace {
@@ -1581,6 +1584,9 @@ void voss_destroy(voss_ctx* c) {
     c->d_mr_v.release();
     c->d_surv.release();
     c->d_nsurv.release();
+    c->d_vf_pa.release();
+    c->d_vf_gap.release();
+    c->d_vf_ok.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -2296,13 +2302,28 @@ int voss_verify_gaps(voss_ctx* c, const uint64_t* p_after, const int32_t* gap, s
     std::vector<uint8_t> v(2 * count);
     int rc = voss_mr_batch(c, ends.data(), ends.size(), v.data());
     if (rc) return rc;
-    // interior: no prime strictly between the ends (:63-92), GPU interval
-    // sieve over (p_before, p_after) for the gaps whose ends are prime
+    // interior: no prime strictly between the ends (:63-92).  Gaps up to
+    // kKernelGap: one device thread per gap tests its interior numbers with
+    // the same Miller-Rabin (verify_interior_kernel); longer ones: the
+    // interval sieve over (p_before, p_after) by a batch run.
+    constexpr int32_t kKernelGap = 1 << 16;
+    for (size_t i = 0; i < count; i++) ok[i] = (uint8_t)(v[2 * i] && v[2 * i + 1]);
+    if (count >= 0xFFFFFFFFull) return fail(VOSS_EINVAL, "count must be below 2^32");
+    if ((rc = set_device(c))) return rc;
+    HIPCHK(c->d_vf_pa.reserve(count));
+    HIPCHK(c->d_vf_gap.reserve(count));
+    HIPCHK(c->d_vf_ok.reserve(count));
+    HIPCHK(hipMemcpyAsync(c->d_vf_pa.p, p_after, count * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_vf_gap.p, gap, count * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_vf_ok.p, ok, count, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_verify_interior(c->d_vf_pa.p, c->d_vf_gap.p, c->d_vf_ok.p, (uint32_t)count,
+                                  kKernelGap, c->stream));
+    HIPCHK(hipMemcpyAsync(ok, c->d_vf_ok.p, count, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     std::vector<uint64_t> lo, hi;
     std::vector<size_t> idx;
     for (size_t i = 0; i < count; i++) {
-        ok[i] = (uint8_t)(v[2 * i] && v[2 * i + 1]);
-        if (ok[i] && gap[i] >= 2) {
+        if (ok[i] && gap[i] >= 2 && (gap[i] > kKernelGap || p_after[i] < (uint64_t)gap[i])) {
             lo.push_back(ends[2 * i] + 1);
             hi.push_back(p_after[i]);
             idx.push_back(i);

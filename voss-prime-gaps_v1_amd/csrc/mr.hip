@@ -243,6 +243,73 @@ __global__ __launch_bounds__(256) void mr_stage2(const uint64_t* __restrict__ ca
     }
 }
 
+// Strong probable-prime test to base b by binary powering (the rare path of
+// verify_interior: only interior numbers that pass base 2 reach it).
+__device__ __forceinline__ bool sprp_binary(const Mont& M, uint64_t bm, uint64_t d, int s) {
+    uint64_t x = bm;
+    for (int k = 62 - __clzll((long long)d); k >= 0; k--) {
+        x = mont_sqr(x, M);
+        if ((d >> k) & 1) x = mont_mul(x, bm, M);
+    }
+    return sprp_tail(M, x, s);
+}
+
+// is_prime_u64 of the reference (src/verify_cpp.cpp:38-57), one thread.
+__device__ __forceinline__ bool is_prime_one(uint64_t n) {
+    const int v = trial(n);
+    if (v != 2) return v == 1;
+    const Mont M = mont_setup(n);
+    uint64_t d = n - 1;
+    const int s = __ffsll((long long)d) - 1;
+    d >>= s;
+    if (!sprp2(M, d, s)) return false;
+    uint64_t bm = add_mod(M.one, M.one, n);  // base 2
+    uint32_t b = 2;
+    for (int t = 0; t < 11; t++) {
+        for (const uint32_t nb = c_bases[t]; b < nb; b++) bm = add_mod(bm, M.one, n);
+        if (!sprp_binary(M, bm, d, s)) return false;
+    }
+    return true;
+}
+
+// The interior of verify_gap (src/verify_cpp.cpp:63-92): with both ends
+// prime, the reference's odd sieve by the primes up to isqrt(p_after) marks
+// every number of (p_before, p_after) exactly when none of them is prime
+// (a composite n < p_after has an odd prime factor <= isqrt(p_after), whose
+// first multiple after p_before the sieve reaches; the only other marks, of
+// a prime q <= isqrt(p_after) inside the interval, need p_after < 7, where
+// both agree).  So one thread per gap tests the interior numbers coprime to
+// 30 (and 3, 5 themselves) with the same Miller-Rabin and clears ok[i] at
+// the first prime.  ok[i] enters as "both ends prime"; gaps above max_gap
+// or above p_after are left to the caller.
+__global__ void verify_interior_kernel(const uint64_t* __restrict__ p_after,
+                                       const int32_t* __restrict__ gap,
+                                       uint8_t* __restrict__ ok, uint32_t count,
+                                       int32_t max_gap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count || !ok[i]) return;
+    const int32_t g = gap[i];
+    const uint64_t pa = p_after[i];
+    if (g < 2 || g > max_gap || pa < (uint64_t)g) return;
+    const uint64_t pb = pa - (uint64_t)g;
+    for (uint64_t n = (pb + 1) | 1ull; n < pa; n += 2) {
+        if (n > 5 && (n % 3ull == 0 || n % 5ull == 0)) continue;
+        if (is_prime_one(n)) {
+            ok[i] = 0;
+            return;
+        }
+    }
+}
+
+hipError_t launch_verify_interior(const uint64_t* d_p_after, const int32_t* d_gap,
+                                  uint8_t* d_ok, uint32_t count, int32_t max_gap,
+                                  hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(verify_interior_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+                       d_p_after, d_gap, d_ok, count, max_gap);
+    return hipGetLastError();
+}
+
 hipError_t launch_mr(const uint64_t* d_cand, uint64_t count, uint8_t* d_verdict,
                      uint32_t* d_surv, uint32_t* d_nsurv, int num_cus,
                      hipStream_t s) {
