@@ -1,7 +1,9 @@
 """Certificates drop-in (src/generate_certificates.py) on a small results/
 tree.  Parity unpinned: the reference ships no certificate fixture and its
 script needs numba; the hashed record format is restated from
-scripts/generate_certificates.py:105-108 and the file layout from :196-233.
+scripts/generate_certificates.py:105-108, which agrees with the format the
+reference's README states (README.md:277-278), and the file layout from
+:196-233.
 What is checked here: rows in file order, verified flags taken from the
 details file, top-20 selection by a stable merit sort, file names."""
 import csv
