@@ -511,3 +511,26 @@ def test_concurrent_contexts_share_the_host_pool(engine):
     assert not errors, errors
     for key, v in got.items():
         assert v == (want_plain if key[0] == "p" else want_batch), key
+
+
+def test_interval_at_the_2_62_bound():
+    """[2^62 - 3e5, 2^62): the largest hi the library accepts, base primes up
+    to 2^31.  The oracle's sieve takes minutes there, so the expected outputs
+    come from the interval's primes as the GPU Miller-Rabin finds them (pinned
+    on its own against a sieve and the oracle) through tests/from_primes.py
+    (pinned against the oracle's sieve).  A context of its own, so the
+    session's does not keep 1.7 GB of base-prime records."""
+    import vossgpu
+    from from_primes import interval_results
+    lo, hi = 2**62 - 300000, 2**62
+    other = vossgpu.Engine(0)
+    try:
+        ns = np.uint64(lo + 1) + np.arange(0, (hi - lo) // 2, dtype=np.uint64) * np.uint64(2)
+        print("[gpu] primes of the interval below 2^62 by Miller-Rabin", flush=True)
+        want = interval_results(ns[other.mr(ns).astype(bool)])
+        print("[gpu] interval below 2^62 (base primes to 2^31)", flush=True)
+        got = other.run_interval(lo, hi, large_cap=4096)
+    finally:
+        other.close()
+    assert want["total_primes"] > 6000
+    assert_same(got, want)

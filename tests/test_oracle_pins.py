@@ -283,3 +283,21 @@ def test_mr_base2_pseudoprimes_above_2_31(oracle):
     assert len(semi) > 500 and len(carm) > 1000 and sum(s for _, s in carm) > 100
     ns = np.array(semi + [n for n, _ in carm], dtype=np.uint64)
     assert not oracle.is_prime_batch(ns).any()
+
+
+@pytest.mark.parametrize("lo,hi", [(10**12, 10**12 + 2 * 10**6), (10**15 + 7, 10**15 + 10**6),
+                                   (49, 100000), (3 * 10**9, 3 * 10**9 + 10**7)])
+def test_outputs_from_a_prime_list(oracle, lo, hi):
+    """tests/from_primes.py (the interval outputs derived from the list of the
+    interval's primes) equals the oracle's sieve: what lets a primality test
+    stand in for the sieve where the oracle is too slow."""
+    from from_primes import interval_results
+    ns = np.arange(lo | 1, hi, 2, dtype=np.uint64)
+    d = interval_results(ns[oracle.is_prime_batch(ns).astype(bool)])
+    r = oracle.run_interval(lo, hi)
+    for k in ("total_primes", "first_prime", "last_prime", "c1", "c3", "large_count"):
+        assert d[k] == r[k], k
+    assert np.array_equal(d["modq"], r["modq"])
+    assert np.array_equal(d["gaps"], r["gaps"])
+    assert np.array_equal(d["modq_gap"], r["modq_gap"])
+    assert d["large"] == [(p, g, float(np.float32(m))) for p, g, m in r["large"]]
