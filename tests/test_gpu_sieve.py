@@ -534,3 +534,31 @@ def test_interval_at_the_2_62_bound():
         other.close()
     assert want["total_primes"] > 6000
     assert_same(got, want)
+
+
+def test_batch_of_high_intervals_matches_primality():
+    """A batch of 12 intervals between 1e17 and 2^62 (six of 1e5 numbers, six
+    of 8e6: three LDS windows each, so their primes from 2^20 to 2^31 go
+    through HBM bucket lists), each against the outputs derived from its
+    primes as the GPU Miller-Rabin finds them (tests/from_primes.py)."""
+    import random as _random
+    import vossgpu
+    from from_primes import interval_results
+    rng = _random.Random(0x564F5353 + 50)
+    iv = []
+    for k in range(12):
+        ln = 10**5 if k % 2 == 0 else 8 * 10**6
+        lo = rng.randrange(10**17, 2**62 - ln)
+        iv.append((lo, lo + ln))
+    other = vossgpu.Engine(0)
+    try:
+        print("[gpu] batch of 12 intervals in [1e17, 2^62)", flush=True)
+        got, _ = other.run_batch([a for a, _ in iv], [b for _, b in iv], large_cap=4096)
+        for (a, b), g in zip(iv, got):
+            ns = np.uint64(a | 1) + np.arange(0, (b - (a | 1) + 1) // 2, dtype=np.uint64) * np.uint64(2)
+            assert int(ns[-1]) < b
+            want = interval_results(ns[other.mr(ns).astype(bool)])
+            assert want["total_primes"] > 2000
+            assert_same(g, want)
+    finally:
+        other.close()
