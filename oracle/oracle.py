@@ -1,8 +1,8 @@
 """ctypes binding of the C oracle (``oracle/libvoss_oracle.so``).
 
 TEST INFRASTRUCTURE ONLY: used by tests/, ``__graft_entry__.smoke()`` and the
-cpu_baseline leg of ``bench.py`` as the checker; the product path never
-imports it.  Each wrapper names the reference function it restates (paths in
+cpu_baseline leg of ``bench.py`` to compare results with; the product path
+never imports it.  Each wrapper names the reference function it restates (paths in
 the reference tree).
 """
 import ctypes
