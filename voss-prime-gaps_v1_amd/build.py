@@ -3,7 +3,7 @@
     python voss-prime-gaps_v1_amd/build.py          # everything
     python voss-prime-gaps_v1_amd/build.py lib      # libvoss_amd.so only
 
-Outputs (in-tree, git-ignored, shipped to the GPU box with the tree):
+Outputs (in-tree and git-ignored; the tree carries them to where it runs):
     voss-prime-gaps_v1_amd/lib/libvoss_amd.so   HIP kernels + C ABI
     voss-prime-gaps_v1_amd/bin/voss_v7_live     reference-compatible sieve CLI
     voss-prime-gaps_v1_amd/bin/verify_cpp       reference-compatible verifier CLI
