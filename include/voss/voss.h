@@ -241,7 +241,7 @@ int voss_mr_batch_device(voss_ctx* ctx, const uint64_t* d_n, size_t count,
 
 /* verify_gap (src/verify_cpp.cpp:59-93) for a batch of gaps: both ends
  * prime (GPU Miller-Rabin) and every interior number composite (gaps up to
- * 65,536: one GPU thread per gap tests its interior with the same
+ * 65,536: one GPU wave per gap tests its interior with the same
  * Miller-Rabin; longer: the GPU interval sieve).  ok[i] = 1 iff gap i
  * verifies. */
 int voss_verify_gaps(voss_ctx* ctx, const uint64_t* p_after,

@@ -52,6 +52,13 @@ def test_verify_gaps(engine, oracle):
     pa = np.array([p for p, _ in rows], dtype=np.uint64)
     g = np.array([x for _, x in rows], dtype=np.int32)
     assert np.array_equal(engine.verify_gaps(pa, g), oracle.verify_gaps(pa, g))
+    # the last primes below 2^64, found by testing every odd number downwards:
+    # consecutive pairs verify, a pair that skips one does not (the interior
+    # ends next to 2^64; the oracle's sieve up to 2^32 is too slow here)
+    top = [n for n in range(2**64 - 1, 2**64 - 400, -2) if oracle.is_prime(n)][:3]
+    pa = np.array([top[0], top[1], top[0]], dtype=np.uint64)
+    g = np.array([top[0] - top[1], top[1] - top[2], top[0] - top[2]], dtype=np.int32)
+    assert engine.verify_gaps(pa, g).tolist() == [1, 1, 0]
 
 
 def test_mr_near_powers_of_two(engine, oracle):

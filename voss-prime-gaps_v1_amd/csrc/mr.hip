@@ -278,24 +278,34 @@ __device__ __forceinline__ bool is_prime_one(uint64_t n) {
 // (a composite n < p_after has an odd prime factor <= isqrt(p_after), whose
 // first multiple after p_before the sieve reaches; the only other marks, of
 // a prime q <= isqrt(p_after) inside the interval, need p_after < 7, where
-// both agree).  So one thread per gap tests the interior numbers coprime to
-// 30 (and 3, 5 themselves) with the same Miller-Rabin and clears ok[i] at
-// the first prime.  ok[i] enters as "both ends prime"; gaps above max_gap
-// or above p_after are left to the caller.
-__global__ void verify_interior_kernel(const uint64_t* __restrict__ p_after,
-                                       const int32_t* __restrict__ gap,
-                                       uint8_t* __restrict__ ok, uint32_t count,
-                                       int32_t max_gap) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count || !ok[i]) return;
+// both agree).  So one wave per gap tests the interior numbers coprime to
+// 30 (and 3, 5 themselves) with the same Miller-Rabin, its lanes on 64
+// consecutive odd numbers at a time, and clears ok[i] at the first prime
+// (a wave vote ends the loop for every lane at once).  ok[i] enters as
+// "both ends prime"; gaps above max_gap or above p_after are left to the
+// caller.
+__global__ __launch_bounds__(256) void verify_interior_kernel(
+    const uint64_t* __restrict__ p_after, const int32_t* __restrict__ gap,
+    uint8_t* __restrict__ ok, uint32_t count, int32_t max_gap) {
+    const uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6);  // one gap per wave
+    const uint32_t lane = threadIdx.x & 63u;
+    if (i >= count || !ok[i]) return;  // uniform over the wave
     const int32_t g = gap[i];
     const uint64_t pa = p_after[i];
     if (g < 2 || g > max_gap || pa < (uint64_t)g) return;
     const uint64_t pb = pa - (uint64_t)g;
-    for (uint64_t n = (pb + 1) | 1ull; n < pa; n += 2) {
-        if (n > 5 && (n % 3ull == 0 || n % 5ull == 0)) continue;
-        if (is_prime_one(n)) {
-            ok[i] = 0;
+    // interior numbers as offsets d from p_before (d < g <= max_gap, so no
+    // sum wraps even next to 2^64); the first odd one is pb + 1 or pb + 2.
+    // The loop bound is uniform over the wave, so every lane reaches the vote.
+    for (uint32_t off = (pb & 1u) ? 2u : 1u; off < (uint32_t)g; off += 128u) {
+        const uint32_t d = off + 2u * lane;
+        bool prime = false;
+        if (d < (uint32_t)g) {
+            const uint64_t n = pb + d;
+            if (n <= 5 || (n % 3ull != 0 && n % 5ull != 0)) prime = is_prime_one(n);
+        }
+        if (__any(prime)) {
+            if (lane == 0) ok[i] = 0;
             return;
         }
     }
@@ -305,7 +315,7 @@ hipError_t launch_verify_interior(const uint64_t* d_p_after, const int32_t* d_ga
                                   uint8_t* d_ok, uint32_t count, int32_t max_gap,
                                   hipStream_t s) {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(verify_interior_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+    hipLaunchKernelGGL(verify_interior_kernel, dim3((count + 3) / 4), dim3(256), 0, s,
                        d_p_after, d_gap, d_ok, count, max_gap);
     return hipGetLastError();
 }
