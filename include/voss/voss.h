@@ -16,7 +16,13 @@
  *     failure on the calling thread.
  *   - Plain pointers and sizes only.  Output buffers are caller-owned.
  *   - A voss_ctx owns device memory, streams and cached base-prime tables
- *     for one GPU.  A context is used by one host thread at a time.
+ *     for one GPU.  One host thread is inside the library at a time per
+ *     process: an entry point called while another thread's call runs
+ *     returns VOSS_EBUSY at once, without touching the device (context
+ *     creation and destruction, voss_batch_free and a server's steps wait
+ *     for the running call instead).  While a server runs, its context
+ *     answers direct calls with VOSS_EBUSY; clients submit requests, which
+ *     is the entry point for concurrent use.
  *   - Integers only on the device; merit is computed on the host with the C
  *     library's log, as float((double)gap / log((double)p_before)).
  */
@@ -35,7 +41,9 @@ extern "C" {
 #define VOSS_EDEVICE (-2) /* HIP runtime / kernel failure                   */
 #define VOSS_ENOMEM (-3)  /* device or host allocation failed               */
 #define VOSS_ERANGE (-4)  /* a gap beyond the device histogram (never < 2^64) */
-#define VOSS_EBUSY (-5)   /* serving queue full / HBM budget exhausted      */
+#define VOSS_EBUSY (-5)   /* another thread is inside the library, the
+                             context belongs to a running server, or the
+                             server is stopping                              */
 #define VOSS_ENOTFOUND (-6) /* unknown serving request id                   */
 
 /* Limits of the reference's output arrays (src/voss_v7_live.cu:23-35). */
@@ -127,10 +135,11 @@ typedef struct {
 
 /* Replaces the reference's per-run setup in main(): cudaGetDeviceProperties,
  * cudaStreamCreate and the cudaMalloc block (src/voss_v7_live.cu:353-401).
- * A context is used by one host thread at a time, and one process drives
- * one context at a time: calls on two contexts from two threads at once
- * are not supported (such a run stopped making progress on an MI355X, cause
- * not found).  The serve API is the entry point for concurrent clients. */
+ * Calls on two contexts from two threads at once are refused (VOSS_EBUSY,
+ * see the conventions above): such a run once stopped making progress on
+ * an MI355X, and device-synchronising calls of one thread (hipFree,
+ * hipHostFree, null-stream copies) may meet the other's stream capture.
+ * The serve API is the entry point for concurrent clients. */
 int voss_create(int device, voss_ctx** out);
 void voss_destroy(voss_ctx* ctx);
 const char* voss_last_error(void);
@@ -240,10 +249,11 @@ int voss_mr_batch_device(voss_ctx* ctx, const uint64_t* d_n, size_t count,
                          uint8_t* d_verdict, double* kernel_ms);
 
 /* verify_gap (src/verify_cpp.cpp:59-93) for a batch of gaps: both ends
- * prime (GPU Miller-Rabin) and every interior number composite (gaps up to
- * 65,536: one GPU wave per gap tests its interior with the same
- * Miller-Rabin; longer: the GPU interval sieve).  ok[i] = 1 iff gap i
- * verifies. */
+ * prime (GPU Miller-Rabin) and every interior number composite (one GPU
+ * wave per gap tests its interior with the same Miller-Rabin and stops at
+ * the first prime).  Rows the reference treats specially get its verdicts:
+ * gap < 2 verifies when both "ends" are prime; a gap larger than p_after
+ * (p_before wraps past 0) never verifies.  ok[i] = 1 iff gap i verifies. */
 int voss_verify_gaps(voss_ctx* ctx, const uint64_t* p_after,
                      const int32_t* gap, size_t count, uint8_t* ok);
 

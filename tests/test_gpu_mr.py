@@ -42,6 +42,12 @@ def test_verify_gaps(engine, oracle):
     rows += [(p, g - 2) for p, g in rows[:40]] + [(p + 2, g + 2) for p, g in rows[:40]]
     rows += [(11, 4), (13, 6), (7, 0), (15, 4), (9, 1), (25056082543, 456), (25056082543, 454)]
     rows += [(3, 1), (5, 3), (7, 5), (7, 2), (5, 2), (29, 27), (37, 2)]
+    # malformed rows, decided row by row as the reference does (the oracle's
+    # verdicts are pinned to the reference binary in test_oracle_ref.py): a
+    # negative gap, and gaps beyond p_after whose p_before wraps to a prime
+    # just below 2^64
+    rows += [(7, -4), (11, -2), (7, -6), (9, -2), (7, 66), (13, 72), (2, 61), (3, 62),
+             (31, 114), (5, 88)]
     # claimed gaps beyond the per-thread interior check (65,536), both ends
     # prime: they go through the interval-sieve path and fail (primes inside)
     for pa in (1000000007, 10**12 + 39):
@@ -59,6 +65,18 @@ def test_verify_gaps(engine, oracle):
     pa = np.array([top[0], top[1], top[0]], dtype=np.uint64)
     g = np.array([top[0] - top[1], top[1] - top[2], top[0] - top[2]], dtype=np.int32)
     assert engine.verify_gaps(pa, g).tolist() == [1, 1, 0]
+    # claimed gaps of 2^16 .. 2^31 - 1 with both ends prime, next to 2^63
+    # and 2^64 (beyond the interval sieve's 2^62 bound): a prime lies inside
+    # every one of them (no prime gap below 2^64 exceeds 1550), so none
+    # verifies, and one malformed row does not fail the others
+    rows = [(top[0], top[0] - top[1])]
+    for pa0 in (top[0], [n for n in range(2**63 + 1, 2**63 + 4000, 2) if oracle.is_prime(n)][0]):
+        for g0 in (65538, 10**6, 2**31 - 4002):  # even: pa0 - g1 stays odd
+            g1 = next(x for x in range(g0, g0 + 4000, 2) if oracle.is_prime(pa0 - x))
+            rows.append((pa0, g1))
+    pa = np.array([p for p, _ in rows], dtype=np.uint64)
+    g = np.array([x for _, x in rows], dtype=np.int32)
+    assert engine.verify_gaps(pa, g).tolist() == [1] + [0] * (len(rows) - 1)
 
 
 def test_mr_near_powers_of_two(engine, oracle):

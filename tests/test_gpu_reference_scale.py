@@ -11,6 +11,7 @@ import csv
 import json
 import os
 
+import numpy as np
 import pytest
 
 from conftest import GOLDEN
@@ -77,6 +78,43 @@ def test_plain_1e14_matches_reference_outputs(engine):
     assert ch[1] == [str(N), str(r["c1"] + 1), str(r["c3"] + 1), str(r["c3"] - r["c1"])]
     hl = read_rows("hl_trend_1e14.csv")
     assert hl[-1] == [str(N), f"{gaps[6] / gaps[2]:.6f}"]
+
+
+def test_gap_sequence_1e14_matches_reference_autocorr(engine):
+    """The gap_sequence.csv sample of [1, 1e14], SEG_NUM = 3e10 (the middle
+    of each of the 3,334 logical segments, src/voss_v7_live.cu:526-547,
+    :235-253), against the reference's own summary of its 1e14 sample,
+    results/autocorr.json, which scripts/autocorrelation.py computed from
+    that file's rows (gap > 0, in index order): sample_size and
+    mean_gap_raw exactly (a float64 sum of integers below 2^53 is exact),
+    mean_normalized, std_normalized and the lag autocorrelations of
+    gap / ln(p) within 1e-9 (numpy's log may differ in the last ulp between
+    machines).  Runs right after the 1e14 test on the same context, whose
+    per-window prime counts the sampler reuses (otherwise it re-runs the
+    1e14 sieve first)."""
+    N = 10**14
+    with open(os.path.join(REF, "autocorr_1e14.json")) as f:
+        ref = json.load(f)
+    assert ref["N"] == N
+    print("[gpu] gap_sequence sample of [1, 1e14] vs the reference's autocorr.json", flush=True)
+    gs, ps, rows = engine.sample_plain(N, SEG_NUM)
+    keep = gs > 0
+    assert rows == int(keep.sum()) == ref["sample_size"] == 4997666
+    gaps = gs[keep].astype(np.float64)
+    pos = ps[keep].astype(np.float64)
+    assert float(gaps.mean()) == ref["mean_gap_raw"]
+    # every row lies in the middle of its logical segment, p_before < p_after
+    idx = np.nonzero(keep)[0]
+    seg = (ps[keep] - 1) // SEG_NUM
+    assert np.array_equal(seg, idx // (5000000 // 3334))
+    norm = gaps / np.log(pos)
+    assert abs(float(norm.mean()) - ref["mean_normalized"]) < 1e-9
+    assert abs(float(norm.std()) - ref["std_normalized"]) < 1e-9
+    c = norm - norm.mean()
+    var = c.var()
+    for lag in (1, 2, 5, 10):
+        ac = float((c[:-lag] * c[lag:]).mean() / var)
+        assert abs(ac - ref[f"ac_lag{lag}"]) < 1e-9, (lag, ac, ref[f"ac_lag{lag}"])
 
 
 def test_large_gap_top_k_equals_full_sort(engine):

@@ -449,9 +449,69 @@ def test_prepared_batch_with_mr(engine):
             L.voss_device_free(engine._h, dv)
 
 
-@pytest.mark.skip(reason="two contexts driven from two host threads at once stopped making "
-                         "progress on an MI355X (no output for 180 s, killed) and no cause was "
-                         "found in this library; such concurrent use is unsupported until one is")
+def test_server_owned_context_refuses_direct_calls(engine):
+    """While a server runs, its context answers direct calls with VOSS_EBUSY
+    (no device work), requests still complete, and after the stop the
+    context is usable again with the same results."""
+    import vossgpu
+    lo, hi = 10**9, 10**9 + 10**6
+    with engine.serve(max_active=4, windows_per_step=64, large_capacity=64) as srv:
+        print("[gpu] direct call on a served context", flush=True)
+        with pytest.raises(vossgpu.VossError) as e:
+            engine.run_plain(10**6, 30000000000)
+        assert e.value.code == -5
+        with pytest.raises(vossgpu.VossError) as e:
+            engine.mr(np.array([7, 9], dtype=np.uint64))
+        assert e.value.code == -5
+        with pytest.raises(vossgpu.VossError) as e:
+            engine.serve(max_active=2)  # one server per context
+        assert e.value.code == -5
+        served, _ = srv.wait(srv.submit_interval(lo, hi))
+    alone = engine.run_batch([lo], [hi], large_cap=64)[0][0]
+    for k in ("total_primes", "first_prime", "last_prime", "c1", "c3", "large_count"):
+        assert served[k] == alone[k]
+    assert np.array_equal(served["gaps"], alone["gaps"])
+
+
+def test_second_thread_is_refused_while_a_call_runs(engine):
+    """A call on another context from a second thread while one thread's
+    call runs returns VOSS_EBUSY at once, before it touches the device; the
+    running call is unaffected, and afterwards the second context works.
+    The two threads never run device work at the same time."""
+    import threading
+    import time
+    import vossgpu
+    other = vossgpu.Engine(0)
+    try:
+        want = other.run_plain(10**7, 30000000000)
+        res = {}
+
+        def long_call():
+            res["r"] = engine.run_plain(10**13, 30000000000, large_cap=16)  # ~2.2 s
+
+        print("[gpu] second thread refused while a 1e13 run holds the library", flush=True)
+        t = threading.Thread(target=long_call)
+        t.start()
+        time.sleep(0.5)
+        code = None
+        try:
+            other.run_plain(10**7, 30000000000)
+        except vossgpu.VossError as e:
+            code = e.code
+        t.join()
+        assert code == -5
+        assert res["r"]["total_primes"] == 346065536839
+        got = other.run_plain(10**7, 30000000000)
+        assert got["total_primes"] == want["total_primes"] == 664579
+        assert np.array_equal(got["gaps"], want["gaps"])
+    finally:
+        other.close()
+
+
+@pytest.mark.skip(reason="the run that stalled (two contexts driven from two host threads with "
+                         "device work overlapping): such calls are now refused with VOSS_EBUSY "
+                         "by the library's call gate, so this body would see refusals; kept as "
+                         "the record of the stalled case")
 def test_concurrent_contexts_share_the_host_pool(engine):
     """Two contexts driven from two host threads at once: their planning,
     accumulation and result steps meet in the one host worker pool (the

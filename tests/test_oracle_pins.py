@@ -264,6 +264,63 @@ def test_mr_mulmod_count(oracle):
     assert oracle.mr_mulmods(n) == 12 * (d.bit_length() + bin(d).count("1"))
 
 
+def witness_squarings(n, a):
+    """Squarings is_prime_u64's witness loop runs for base a
+    (src/verify_cpp.cpp:47-54): none when a^d is 1 or -1, else one per step
+    until -1 appears or r - 1 steps are spent; and whether a passes."""
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        r += 1
+    x = pow(a, d, n)
+    if x in (1, n - 1):
+        return 0, True
+    for j in range(1, r):
+        x = x * x % n
+        if x == n - 1:
+            return j, True
+    return r - 1, False
+
+
+@pytest.mark.parametrize("n,stop_base", [
+    (41, None),                    # prime, r = 3: witness loops of 0..2 squarings
+    (998244353, None),             # prime 119 * 2^23 + 1, r = 23
+    (2**64 - 59, None),            # prime, r = 1
+    (1373653, 5),                  # psi_2 = 829 * 1657: passes 2 and 3, fails at 5
+    (25326001, 7),                 # psi_3: passes 2, 3, 5, fails at 7
+    (3825123056546413051, 37),     # psi_9..psi_11: passes 2..31, fails at 37
+    (341, 2),                      # 11 * 31: trial division (11 is a base)
+    (4033, None),                  # 37 * 109, a base-2 strong pseudoprime: trial (37)
+])
+def test_mr_mulmod_count_witness_loop(oracle, n, stop_base):
+    """The count with the witness-loop squarings, on primes with r > 1 and
+    on strong pseudoprimes that pass some bases before one rejects them:
+    per base tried, bitlen(d) + popcount(d) products of mod_pow plus the
+    squarings that base's loop runs (each base's loop counted from the
+    base's own power, so a dropped or extra squaring in any base fails)."""
+    bases = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37)
+    if any(n % b == 0 for b in bases):
+        assert oracle.mr_mulmods(n) == 0
+        return
+    d = n - 1
+    while d % 2 == 0:
+        d //= 2
+    per = d.bit_length() + bin(d).count("1")
+    want = 0
+    for a in bases:
+        sq, ok = witness_squarings(n, a)
+        want += per + sq
+        if not ok:
+            assert a == stop_base, (n, a)
+            break
+    else:
+        assert stop_base is None
+    assert oracle.mr_mulmods(n) == want
+    # at least one tested number exercises loops of 2+ squarings
+    if n == 998244353:
+        assert max(witness_squarings(n, a)[0] for a in bases) >= 2
+
+
 def test_verify_gap_semantics(oracle):
     assert oracle.verify_gap(11, 4)          # 7 -> 11
     assert not oracle.verify_gap(13, 6)      # 7 -> 13 skips 11
@@ -291,13 +348,16 @@ def test_outputs_from_a_prime_list(oracle, lo, hi):
     """tests/from_primes.py (the interval outputs derived from the list of the
     interval's primes) equals the oracle's sieve: what lets a primality test
     stand in for the sieve where the oracle is too slow."""
-    from from_primes import interval_results
+    from from_primes import interval_results, interval_results_np
     ns = np.arange(lo | 1, hi, 2, dtype=np.uint64)
-    d = interval_results(ns[oracle.is_prime_batch(ns).astype(bool)])
+    ps = ns[oracle.is_prime_batch(ns).astype(bool)]
     r = oracle.run_interval(lo, hi)
-    for k in ("total_primes", "first_prime", "last_prime", "c1", "c3", "large_count"):
-        assert d[k] == r[k], k
-    assert np.array_equal(d["modq"], r["modq"])
-    assert np.array_equal(d["gaps"], r["gaps"])
-    assert np.array_equal(d["modq_gap"], r["modq_gap"])
-    assert d["large"] == [(p, g, float(np.float32(m))) for p, g, m in r["large"]]
+    for d in (interval_results(ps), interval_results_np(ps)):
+        for k in ("total_primes", "first_prime", "last_prime", "c1", "c3", "large_count"):
+            assert d[k] == r[k], k
+        assert np.array_equal(d["modq"], r["modq"])
+        assert np.array_equal(d["gaps"], r["gaps"])
+        assert np.array_equal(d["modq_gap"], r["modq_gap"])
+        assert d["large"] == [(p, g, float(np.float32(m))) for p, g, m in r["large"]]
+    if lo == 3 * 10**9:
+        assert r["large_count"] > 0  # the merit rule is exercised

@@ -48,6 +48,11 @@ def test_mr_and_verify_gap_match_reference_binary(oracle, tmp_path):
     # wrong gaps: shifted ends, a gap that skips a prime, odd gaps
     gaps += [(p, g - 2) for p, g in gaps[:50]] + [(p + 2, g + 2) for p, g in gaps[:50]]
     gaps += [(11, 4), (13, 6), (7, 0), (15, 4), (9, 1)]
+    # malformed rows: a negative gap (p_before = p_after + |gap|, "verified"
+    # when both are prime: gap < 2), and gaps beyond p_after whose p_before
+    # wraps to a prime just below 2^64 (2^64 - 59, 2^64 - 83)
+    gaps += [(7, -4), (11, -2), (7, -6), (9, -2), (7, 66), (13, 72), (2, 61), (3, 62),
+             (31, 114), (5, 88)]
     ref = run_ref(tmp_path, [(n, 0) for n in ns] + gaps)
     ref_mr, ref_gap = ref[:len(ns)], ref[len(ns):]
     mine_mr = oracle.is_prime_batch(np.array(ns, dtype=np.uint64))

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -16,6 +17,7 @@
 #include <thread>
 #include <vector>
 
+#include "call_gate.h"
 #include "mr_launch.h"
 #include "sieve_launch.h"
 #include "voss/voss.h"
@@ -342,6 +344,7 @@ struct voss_ctx {
     DescVec desc_pool;
     std::vector<hipEvent_t> evs;  // per-round timing events
     int graph_mode = 1;           // batched runs as one captured hipGraph
+    std::atomic<bool> served{false};  // a running server's scheduler owns it
 
     // per-window prime counts of the last plain run (gap-sequence sampling)
     uint64_t last_N = 0, last_seg = 0;
@@ -360,6 +363,46 @@ struct voss_ctx {
 };
 
 namespace {
+
+// The process-wide gate of library calls (call_gate.h), and whether the
+// calling thread is a server's scheduler.
+CallGate& gate() {
+    static CallGate g;
+    return g;
+}
+thread_local bool t_scheduler = false;
+
+// Admission of one entry point: VOSS_EBUSY when another thread is inside the
+// library, or when the context belongs to a running server (to anyone but
+// that server's scheduler thread).  wait = true blocks for the gate instead
+// (context setup and teardown).
+struct Admit {
+    int rc = VOSS_OK;
+    bool held = false;
+    explicit Admit(const voss_ctx* c, bool wait = false) {
+        if (c && c->served.load() && !t_scheduler) {
+            rc = fail(VOSS_EBUSY, "the context belongs to a running server (submit requests to it)");
+            return;
+        }
+        if (wait) {
+            gate().enter();
+        } else if (!gate().try_enter()) {
+            rc = fail(VOSS_EBUSY,
+                      "another thread is inside the library: one call at a time per process");
+            return;
+        }
+        held = true;
+    }
+    ~Admit() {
+        if (held) gate().leave();
+    }
+    Admit(const Admit&) = delete;
+    Admit& operator=(const Admit&) = delete;
+};
+
+#define VOSS_ADMIT(ctx)          \
+    Admit admit_(ctx);           \
+    if (admit_.rc) return admit_.rc
 
 int set_device(voss_ctx* c) {
     HIPCHK(hipSetDevice(c->device));
@@ -1533,6 +1576,7 @@ extern "C" {
 const char* voss_last_error(void) { return g_err.c_str(); }
 
 int voss_create(int device, voss_ctx** out) {
+    Admit admit_(nullptr, true);
     if (!out) return fail(VOSS_EINVAL, "out is NULL");
     *out = nullptr;
     int ndev = 0;
@@ -1570,6 +1614,7 @@ int voss_create(int device, voss_ctx** out) {
 
 void voss_destroy(voss_ctx* c) {
     if (!c) return;
+    Admit admit_(nullptr, true);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->d_primes.release();
@@ -1605,6 +1650,7 @@ int voss_device_name(voss_ctx* c, char* buf, size_t cap, int* num_cus, double* t
 }
 
 int voss_set_tuning(voss_ctx* c, const voss_tuning* t) {
+    VOSS_ADMIT(c);
     if (!c || !t) return fail(VOSS_EINVAL, "NULL argument");
     voss_tuning n = c->tune;
     if (t->window_bytes) {
@@ -1630,6 +1676,7 @@ int voss_set_tuning(voss_ctx* c, const voss_tuning* t) {
 
 int voss_autotune(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
                   size_t max_intervals, int repeats, voss_tuning* best, double* best_seconds) {
+    VOSS_ADMIT(c);
     if (!c || (n && (!lo || !hi))) return fail(VOSS_EINVAL, "NULL argument");
     if (n == 0) return fail(VOSS_EINVAL, "no intervals to tune on");
     if (repeats < 1) repeats = 1;
@@ -1958,6 +2005,7 @@ void stamp_times(voss_result* out, const ExecTimes& tm, double t0, uint64_t nwin
 extern "C" {
 
 int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) {
+    VOSS_ADMIT(c);
     int rc = plain_args(c, N, seg_num, out);
     if (rc) return rc;
     const double t0 = now_s();
@@ -1987,6 +2035,7 @@ int voss_run_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, voss_result* out) 
 
 int voss_run_plain_range(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t seg_begin,
                          uint64_t seg_end, voss_result* out) {
+    VOSS_ADMIT(c);
     int rc = plain_args(c, N, seg_num, out);
     if (rc) return rc;
     const uint64_t num_seg = (N + seg_num - 1) / seg_num;
@@ -2011,6 +2060,7 @@ int voss_run_plain_range(voss_ctx* c, uint64_t N, uint64_t seg_num, uint64_t seg
 }
 
 int voss_run_interval(voss_ctx* c, uint64_t lo, uint64_t hi, voss_result* out) {
+    VOSS_ADMIT(c);
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     int rc = check_result(out);
     if (rc) return rc;
@@ -2043,11 +2093,13 @@ int voss_run_interval(voss_ctx* c, uint64_t lo, uint64_t hi, voss_result* out) {
 
 int voss_run_batch(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
                    voss_interval_result* out, voss_result* stats) {
+    VOSS_ADMIT(c);
     return run_batch(c, lo, hi, n, out, stats, c ? c->graph_mode != 0 : false);
 }
 
 int voss_run_batch_plain(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
                          voss_interval_result* out, voss_result* stats) {
+    VOSS_ADMIT(c);
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     if (n && (!lo || !hi || !out)) return fail(VOSS_EINVAL, "NULL argument");
     const double t0 = now_s();
@@ -2090,6 +2142,7 @@ extern "C" {
 
 int voss_batch_prepare(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
                        voss_batch** out) {
+    VOSS_ADMIT(c);
     if (!out) return fail(VOSS_EINVAL, "out is NULL");
     *out = nullptr;
     int rc = batch_args(c, lo, hi, n);
@@ -2108,6 +2161,7 @@ int voss_batch_prepare(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size
 }
 
 int voss_batch_run(voss_batch* b, voss_interval_result* out, voss_result* stats) {
+    VOSS_ADMIT(b ? b->c : nullptr);
     if (!b) return fail(VOSS_EINVAL, "batch is NULL");
     voss_ctx* c = b->c;
     const size_t n = b->lo.size();
@@ -2131,6 +2185,7 @@ int voss_batch_run(voss_batch* b, voss_interval_result* out, voss_result* stats)
 }
 
 int voss_batch_add_mr(voss_batch* b, const uint64_t* d_n, size_t count, uint8_t* d_verdict) {
+    VOSS_ADMIT(b ? b->c : nullptr);
     if (!b) return fail(VOSS_EINVAL, "batch is NULL");
     if (count && (!d_n || !d_verdict)) return fail(VOSS_EINVAL, "NULL buffer");
     if (count >= 0xFFFFFFFFull) return fail(VOSS_EINVAL, "count must be below 2^32");
@@ -2147,6 +2202,7 @@ int voss_batch_add_mr(voss_batch* b, const uint64_t* d_n, size_t count, uint8_t*
 
 void voss_batch_free(voss_batch* b) {
     if (!b) return;
+    Admit admit_(nullptr, true);
     (void)hipSetDevice(b->c->device);
     (void)hipStreamSynchronize(b->c->stream);
     b->X.reset();
@@ -2155,6 +2211,7 @@ void voss_batch_free(voss_batch* b) {
 }
 
 int voss_set_graph_mode(voss_ctx* c, int enable) {
+    VOSS_ADMIT(c);
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     c->graph_mode = enable ? 1 : 0;
     return VOSS_OK;
@@ -2162,6 +2219,7 @@ int voss_set_graph_mode(voss_ctx* c, int enable) {
 
 int voss_sample_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, int32_t* gap_seq,
                       uint64_t* pos_seq, uint64_t* rows_out) {
+    VOSS_ADMIT(c);
     if (!gap_seq || !pos_seq) return fail(VOSS_EINVAL, "NULL buffer");
     int rc;
     const uint32_t W = c ? c->tune.window_bytes : 0;
@@ -2253,6 +2311,7 @@ int voss_sample_plain(voss_ctx* c, uint64_t N, uint64_t seg_num, int32_t* gap_se
 
 int voss_mr_batch_device(voss_ctx* c, const uint64_t* d_n, size_t count, uint8_t* d_verdict,
                          double* kernel_ms) {
+    VOSS_ADMIT(c);
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     if (count && (!d_n || !d_verdict)) return fail(VOSS_EINVAL, "NULL buffer");
     if (count >= 0xFFFFFFFFull) return fail(VOSS_EINVAL, "count must be below 2^32");
@@ -2275,6 +2334,7 @@ int voss_mr_batch_device(voss_ctx* c, const uint64_t* d_n, size_t count, uint8_t
 }
 
 int voss_mr_batch(voss_ctx* c, const uint64_t* n, size_t count, uint8_t* verdict) {
+    VOSS_ADMIT(c);
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     if (count && (!n || !verdict)) return fail(VOSS_EINVAL, "NULL buffer");
     int rc;
@@ -2290,6 +2350,7 @@ int voss_mr_batch(voss_ctx* c, const uint64_t* n, size_t count, uint8_t* verdict
 
 int voss_verify_gaps(voss_ctx* c, const uint64_t* p_after, const int32_t* gap, size_t count,
                      uint8_t* ok) {
+    VOSS_ADMIT(c);
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     if (count && (!p_after || !gap || !ok)) return fail(VOSS_EINVAL, "NULL buffer");
     if (count == 0) return VOSS_OK;
@@ -2302,11 +2363,10 @@ int voss_verify_gaps(voss_ctx* c, const uint64_t* p_after, const int32_t* gap, s
     std::vector<uint8_t> v(2 * count);
     int rc = voss_mr_batch(c, ends.data(), ends.size(), v.data());
     if (rc) return rc;
-    // interior: no prime strictly between the ends (:63-92).  Gaps up to
-    // kKernelGap: one device thread per gap tests its interior numbers with
-    // the same Miller-Rabin (verify_interior_kernel); longer ones: the
-    // interval sieve over (p_before, p_after) by a batch run.
-    constexpr int32_t kKernelGap = 1 << 16;
+    // interior: no prime strictly between the ends (:63-92), decided on the
+    // device for every row (verify_interior_kernel: one wave per gap tests
+    // its interior numbers with the same Miller-Rabin; claimed gaps beyond
+    // p_after fail, as in the reference)
     for (size_t i = 0; i < count; i++) ok[i] = (uint8_t)(v[2 * i] && v[2 * i + 1]);
     if (count >= 0xFFFFFFFFull) return fail(VOSS_EINVAL, "count must be below 2^32");
     if ((rc = set_device(c))) return rc;
@@ -2317,32 +2377,14 @@ int voss_verify_gaps(voss_ctx* c, const uint64_t* p_after, const int32_t* gap, s
     HIPCHK(hipMemcpyAsync(c->d_vf_gap.p, gap, count * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_vf_ok.p, ok, count, hipMemcpyHostToDevice, c->stream));
     HIPCHK(launch_verify_interior(c->d_vf_pa.p, c->d_vf_gap.p, c->d_vf_ok.p, (uint32_t)count,
-                                  kKernelGap, c->stream));
+                                  c->stream));
     HIPCHK(hipMemcpyAsync(ok, c->d_vf_ok.p, count, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<uint64_t> lo, hi;
-    std::vector<size_t> idx;
-    for (size_t i = 0; i < count; i++) {
-        if (ok[i] && gap[i] >= 2 && (gap[i] > kKernelGap || p_after[i] < (uint64_t)gap[i])) {
-            lo.push_back(ends[2 * i] + 1);
-            hi.push_back(p_after[i]);
-            idx.push_back(i);
-        }
-    }
-    const size_t kChunk = 4096;
-    std::vector<voss_interval_result> res;
-    for (size_t off = 0; off < lo.size(); off += kChunk) {
-        const size_t m = std::min(kChunk, lo.size() - off);
-        res.assign(m, voss_interval_result());
-        if ((rc = voss_run_batch(c, lo.data() + off, hi.data() + off, m, res.data(), nullptr)))
-            return rc;
-        for (size_t k = 0; k < m; k++)
-            if (res[k].total_primes != 0) ok[idx[off + k]] = 0;
-    }
     return VOSS_OK;
 }
 
 int voss_device_alloc(voss_ctx* c, size_t bytes, void** dptr) {
+    VOSS_ADMIT(c);
     if (!c || !dptr) return fail(VOSS_EINVAL, "NULL argument");
     int rc = set_device(c);
     if (rc) return rc;
@@ -2351,6 +2393,7 @@ int voss_device_alloc(voss_ctx* c, size_t bytes, void** dptr) {
 }
 
 int voss_device_free(voss_ctx* c, void* dptr) {
+    VOSS_ADMIT(c);
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     int rc = set_device(c);
     if (rc) return rc;
@@ -2359,6 +2402,7 @@ int voss_device_free(voss_ctx* c, void* dptr) {
 }
 
 int voss_memcpy_h2d(voss_ctx* c, void* dst, const void* src, size_t bytes) {
+    VOSS_ADMIT(c);
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     int rc = set_device(c);
     if (rc) return rc;
@@ -2367,6 +2411,7 @@ int voss_memcpy_h2d(voss_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 int voss_memcpy_d2h(voss_ctx* c, void* dst, const void* src, size_t bytes) {
+    VOSS_ADMIT(c);
     if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
     int rc = set_device(c);
     if (rc) return rc;
@@ -2506,6 +2551,7 @@ void serve_fail_all(voss_server* s, int code, const std::string& msg) {
 
 void serve_loop(voss_server* s) {
     voss_ctx* c = s->c;
+    t_scheduler = true;
     if (hipSetDevice(c->device) != hipSuccess) {
         serve_fail_all(s, VOSS_EDEVICE, "hipSetDevice");
         return;
@@ -2541,6 +2587,12 @@ void serve_loop(voss_server* s) {
                 s->queue.pop_front();
             }
         }
+        // the step's device work holds the process-wide call gate, so no
+        // other thread's call runs beside it (it waits for one to finish)
+        struct StepGate {
+            StepGate() { gate().enter(); }
+            ~StepGate() { gate().leave(); }
+        } step_gate;
         int rc = VOSS_OK;
         // plan the admitted interval requests; fresh accumulator slots
         for (auto& r : admitted_iv) {
@@ -2671,6 +2723,7 @@ extern "C" {
 int voss_serve_start(voss_ctx* c, const voss_serve_config* cfg, voss_server** out) {
     if (!c || !out) return fail(VOSS_EINVAL, "NULL argument");
     *out = nullptr;
+    VOSS_ADMIT(c);  // also refuses a context that already has a server
     int rc = set_device(c);
     if (rc) return rc;
     voss_server* s = new voss_server();
@@ -2691,6 +2744,7 @@ int voss_serve_start(voss_ctx* c, const voss_serve_config* cfg, voss_server** ou
         return hip_fail(e, "serve setup");
     }
     for (int i = (int)d.max_active - 1; i >= 0; i--) s->free_slots.push_back(i);
+    c->served = true;  // direct calls on c now return VOSS_EBUSY
     s->th = std::thread(serve_loop, s);
     *out = s;
     return VOSS_OK;
@@ -2763,6 +2817,8 @@ int voss_serve_stop(voss_server* s) {
     }
     if (s->th.joinable()) s->th.join();
     const int err = s->err;
+    Admit admit_(nullptr, true);
+    s->c->served = false;
     (void)hipSetDevice(s->c->device);
     if (s->mr_stream) (void)hipStreamDestroy(s->mr_stream);
     s->d_acc.release();

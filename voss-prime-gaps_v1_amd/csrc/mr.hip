@@ -281,22 +281,34 @@ __device__ __forceinline__ bool is_prime_one(uint64_t n) {
 // both agree).  So one wave per gap tests the interior numbers coprime to
 // 30 (and 3, 5 themselves) with the same Miller-Rabin, its lanes on 64
 // consecutive odd numbers at a time, and clears ok[i] at the first prime
-// (a wave vote ends the loop for every lane at once).  ok[i] enters as
-// "both ends prime"; gaps above max_gap or above p_after are left to the
-// caller.
+// (a wave vote ends the loop for every lane at once).  Every gap between
+// primes below 2^64 is at most 1550, so a wave stops within 13 rounds
+// whatever gap it is given.
+// A claimed gap larger than p_after (p_before wraps below 0 to near 2^64,
+// both ends prime) fails in the reference: its sieve runs over the numbers
+// p_before + 1 .. 2^64 - 1, 0, 1, .., p_after - 1 in that order, and the
+// number 1 there sits at the odd index gap - p_after; only the multiples of
+// an odd q <= isqrt(p_after) < 46341 whose first multiple after p_before
+// passes 2^64 are marked, starting from (that multiple - 2^64) or that plus
+// q, which is 1 only if q divides 2^64 + 1 = 274177 * 67280421310721.  So
+// such rows are cleared here.  ok[i] enters as "both ends prime".
 __global__ __launch_bounds__(256) void verify_interior_kernel(
     const uint64_t* __restrict__ p_after, const int32_t* __restrict__ gap,
-    uint8_t* __restrict__ ok, uint32_t count, int32_t max_gap) {
+    uint8_t* __restrict__ ok, uint32_t count) {
     const uint32_t i = blockIdx.x * 4u + (threadIdx.x >> 6);  // one gap per wave
     const uint32_t lane = threadIdx.x & 63u;
     if (i >= count || !ok[i]) return;  // uniform over the wave
     const int32_t g = gap[i];
     const uint64_t pa = p_after[i];
-    if (g < 2 || g > max_gap || pa < (uint64_t)g) return;
+    if (g < 2) return;
+    if (pa < (uint64_t)g) {
+        if (lane == 0) ok[i] = 0;
+        return;
+    }
     const uint64_t pb = pa - (uint64_t)g;
-    // interior numbers as offsets d from p_before (d < g <= max_gap, so no
-    // sum wraps even next to 2^64); the first odd one is pb + 1 or pb + 2.
-    // The loop bound is uniform over the wave, so every lane reaches the vote.
+    // interior numbers as offsets d from p_before (d < g < 2^31, so no sum
+    // wraps even next to 2^64); the first odd one is pb + 1 or pb + 2.  The
+    // loop bound is uniform over the wave, so every lane reaches the vote.
     for (uint32_t off = (pb & 1u) ? 2u : 1u; off < (uint32_t)g; off += 128u) {
         const uint32_t d = off + 2u * lane;
         bool prime = false;
@@ -312,11 +324,10 @@ __global__ __launch_bounds__(256) void verify_interior_kernel(
 }
 
 hipError_t launch_verify_interior(const uint64_t* d_p_after, const int32_t* d_gap,
-                                  uint8_t* d_ok, uint32_t count, int32_t max_gap,
-                                  hipStream_t s) {
+                                  uint8_t* d_ok, uint32_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(verify_interior_kernel, dim3((count + 3) / 4), dim3(256), 0, s,
-                       d_p_after, d_gap, d_ok, count, max_gap);
+                       d_p_after, d_gap, d_ok, count);
     return hipGetLastError();
 }
 

@@ -12,11 +12,11 @@ hipError_t launch_mr(const uint64_t* d_cand, uint64_t count, uint8_t* d_verdict,
                      uint32_t* d_surv, uint32_t* d_nsurv, int num_cus,
                      hipStream_t s);
 
-// verify_gap's interior check for gaps 2..max_gap whose ok[i] is set (both
+// verify_gap's interior check for the gaps >= 2 whose ok[i] is set (both
 // ends prime): ok[i] is cleared when a prime lies strictly between
-// p_after[i] - gap[i] and p_after[i].  count must be below 2^32.
+// p_after[i] - gap[i] and p_after[i], or when gap[i] > p_after[i] (the
+// reference's verdict on such rows).  count must be below 2^32.
 hipError_t launch_verify_interior(const uint64_t* d_p_after, const int32_t* d_gap,
-                                  uint8_t* d_ok, uint32_t count, int32_t max_gap,
-                                  hipStream_t s);
+                                  uint8_t* d_ok, uint32_t count, hipStream_t s);
 
 }  // namespace voss
