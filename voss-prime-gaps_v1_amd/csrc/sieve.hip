@@ -226,6 +226,11 @@ __device__ __forceinline__ void pair_wide(const PairCtx& c, uint32_t pb, uint32_
 template <int WIN, int BLOCK, bool EMIT>
 __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
     constexpr int W4 = WIN / 4;
+    // The 128 KiB window keeps the presieve tables in LDS (one workgroup per
+    // CU).  Smaller windows read them from global memory (14.6 KB, L1
+    // resident) so that two 64 KiB or four 32 KiB workgroups fit in a CU's
+    // LDS and one window's marking (LDS-bound) overlaps another's walk.
+    constexpr bool kPatLds = WIN > 65536;
     __shared__ __align__(16) uint32_t bm[W4];
     __shared__ uint32_t hist[kHistBins];
     // q=30 rows interleaved: mq[8*(g/2) + row] for gaps g < 100; bin 0 takes
@@ -234,7 +239,7 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
     // walk: how many nonzero wheel bytes of each value were seen (their
     // internal prime pairs go into mq at the flush)
     __shared__ uint32_t s_bytes[EMIT ? 1 : 256];
-    __shared__ uint32_t s_pat[kPresieveWordsPadded];
+    __shared__ uint32_t s_pat[kPatLds ? kPresieveWordsPadded : 1];
     __shared__ uint32_t tfirst[BLOCK];
     __shared__ uint32_t tlast[BLOCK];
     __shared__ unsigned long long s_cnt[10];
@@ -250,7 +255,9 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
     for (int i = tid; i < 8 * 64; i += BLOCK) mq[i] = 0;
     if constexpr (!EMIT)
         for (int i = tid; i < 256; i += BLOCK) s_bytes[i] = 0;
-    for (int i = tid; i < kPresieveWordsPadded; i += BLOCK) s_pat[i] = a.pat[i];
+    if constexpr (kPatLds)
+        for (int i = tid; i < kPresieveWordsPadded; i += BLOCK) s_pat[i] = a.pat[i];
+    const uint32_t* const pat = kPatLds ? s_pat : a.pat;
     for (int i = tid; i < kLgtCount; i += BLOCK) s_lgt[i] = a.lg_thresh[kLgtFirst + i];
     uint32_t cur_job = blockIdx.x < a.ndesc ? a.desc[blockIdx.x].job : 0;
 
@@ -336,7 +343,7 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                 uint32_t w0 = 0xFFFFFFFFu, w1 = 0xFFFFFFFFu;
 #pragma unroll
                 for (int g = 0; g < kPresieveGroups; g++) {
-                    const uint32_t* t = s_pat + a.pat_off[g] + idx[g];
+                    const uint32_t* t = pat + a.pat_off[g] + idx[g];
                     w0 &= t[0];
                     w1 &= t[1];
                     const uint32_t v = idx[g] + a.pat_step[g];
