@@ -248,6 +248,20 @@ int voss_mr_batch(voss_ctx* ctx, const uint64_t* n, size_t count,
 int voss_mr_batch_device(voss_ctx* ctx, const uint64_t* d_n, size_t count,
                          uint8_t* d_verdict, double* kernel_ms);
 
+/* No counterpart in the reference.  voss_mr_batch_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same kernels and
+ * verdicts, with the survivor lists and counters in a fixed device array
+ * of the library (32 MiB per GPU, kept until the process ends) instead of
+ * context buffers that grow.  No hipMalloc, hipFree or other allocation.
+ * count > VOSS_MR_NOALLOC_MAX returns VOSS_EINVAL.  Every call on one GPU
+ * shares that array: calls that overlapped on one GPU would race (the
+ * library admits one call at a time per process, and the call returns
+ * after its kernels have finished). */
+#define VOSS_MR_NOALLOC_MAX 4194304
+int voss_mr_batch_device_noalloc(voss_ctx* ctx, const uint64_t* d_n, size_t count,
+                                 uint8_t* d_verdict, double* kernel_ms);
+
 /* verify_gap (src/verify_cpp.cpp:59-93) for a batch of gaps: both ends
  * prime (GPU Miller-Rabin) and every interior number composite (one GPU
  * wave per gap tests its interior with the same Miller-Rabin and stops at

@@ -357,6 +357,8 @@ struct voss_ctx {
     DevBuf<uint8_t> d_mr_v;
     DevBuf<uint32_t> d_surv;
     DevBuf<uint32_t> d_nsurv;
+    uint32_t* mr_scratch = nullptr;  // the no-allocation MR entry point's lists
+                                     // and counters (device address, fixed)
     DevBuf<uint64_t> d_vf_pa;   // verify_gaps: p_after, gap, ok
     DevBuf<int32_t> d_vf_gap;
     DevBuf<uint8_t> d_vf_ok;
@@ -1600,6 +1602,8 @@ int voss_create(int device, voss_ctx** out) {
         if (e == hipSuccess) e = hipEventCreate(&c->ev2);
         if (e == hipSuccess) e = upload_sieve_constants(device);
         if (e != hipSuccess) rc = hip_fail(e, "context setup");
+        if (rc == VOSS_OK && (e = mr_scratch_address(device, &c->mr_scratch)) != hipSuccess)
+            rc = hip_fail(e, "hipGetSymbolAddress(scratch)");
     }
     if (rc == VOSS_OK) rc = build_patterns(c);
     if (rc == VOSS_OK) rc = build_lg_thresh(c);
@@ -2325,6 +2329,33 @@ int voss_mr_batch_device(voss_ctx* c, const uint64_t* d_n, size_t count, uint8_t
     HIPCHK(c->d_nsurv.reserve(3));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(launch_mr(d_n, count, d_verdict, c->d_surv.p, c->d_nsurv.p, c->num_cus, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    return VOSS_OK;
+}
+
+int voss_mr_batch_device_noalloc(voss_ctx* c, const uint64_t* d_n, size_t count,
+                                 uint8_t* d_verdict, double* kernel_ms) {
+    VOSS_ADMIT(c);
+    if (!c) return fail(VOSS_EINVAL, "ctx is NULL");
+    if (count && (!d_n || !d_verdict)) return fail(VOSS_EINVAL, "NULL buffer");
+    if (count > kMrNoAllocMax)
+        return fail(VOSS_EINVAL, "count beyond VOSS_MR_NOALLOC_MAX (4194304) candidates");
+    if (!c->mr_scratch) return fail(VOSS_EDEVICE, "no scratch address for this device");
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    if (count == 0) {
+        if (kernel_ms) *kernel_ms = 0;
+        return VOSS_OK;
+    }
+    // the fixed array instead of c->d_surv / c->d_nsurv: launch_mr clears the
+    // counters with a fill kernel, and the lists are written before read
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(launch_mr(d_n, count, d_verdict, c->mr_scratch, c->mr_scratch + 2 * kMrNoAllocMax,
+                     c->num_cus, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     float ms = 0;

@@ -14,6 +14,8 @@
 // primes and so all take the same number of steps.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "montgomery.h"
 #include "mr_launch.h"
 #include "sieve_launch.h"
@@ -329,6 +331,31 @@ hipError_t launch_verify_interior(const uint64_t* d_p_after, const int32_t* d_ga
     hipLaunchKernelGGL(verify_interior_kernel, dim3((count + 3) / 4), dim3(256), 0, s,
                        d_p_after, d_gap, d_ok, count);
     return hipGetLastError();
+}
+
+// Scratch of the Miller-Rabin entry point that allocates nothing: the two
+// survivor lists of up to kMrNoAllocMax candidates, then the three list
+// counters.  One copy per GPU, held until the process ends (32 MiB).  Host
+// code never passes the array's name to a kernel: that is a host-side
+// stand-in, not the device address; mr_scratch_address() gives the address.
+static __device__ __align__(256) uint32_t scratch[2 * kMrNoAllocMax + 4];
+
+hipError_t mr_scratch_address(int device, uint32_t** out) {
+    // looked up once per device, with that device current (the caller's
+    // context setup), and kept: each GPU holds its own copy of the array
+    static std::mutex mu;
+    static uint32_t* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = addr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_mr(const uint64_t* d_cand, uint64_t count, uint8_t* d_verdict,

@@ -133,7 +133,7 @@ EXPORTS = [
     "voss_set_tuning", "voss_get_tuning", "voss_autotune", "voss_run_plain", "voss_run_plain_range",
     "voss_sample_plain",
     "voss_run_interval", "voss_run_batch", "voss_run_batch_plain", "voss_mr_batch",
-    "voss_mr_batch_device", "voss_verify_gaps", "voss_device_alloc", "voss_device_free",
+    "voss_mr_batch_device", "voss_mr_batch_device_noalloc", "voss_verify_gaps", "voss_device_alloc", "voss_device_free",
     "voss_memcpy_h2d", "voss_memcpy_d2h", "voss_serve_start", "voss_serve_submit_interval",
     "voss_serve_submit_mr", "voss_serve_poll", "voss_serve_wait", "voss_serve_stop",
     "voss_set_graph_mode", "voss_batch_prepare", "voss_batch_run", "voss_batch_free",
@@ -175,6 +175,7 @@ def lib():
         L.voss_batch_free.restype = None
         L.voss_mr_batch.argtypes = [vp, vp, sz, vp]
         L.voss_mr_batch_device.argtypes = [vp, vp, sz, vp, P(ctypes.c_double)]
+        L.voss_mr_batch_device_noalloc.argtypes = [vp, vp, sz, vp, P(ctypes.c_double)]
         L.voss_verify_gaps.argtypes = [vp, vp, vp, sz, vp]
         L.voss_device_alloc.argtypes = [vp, sz, P(vp)]
         L.voss_device_free.argtypes = [vp, vp]
