@@ -17,12 +17,15 @@
 //            Waves pull work from an LDS counter, smallest primes first,
 //            so the phase is balanced without a static split;
 //   phase 2  each thread walks its contiguous run of words in increasing
-//            order: popcounts give the per-residue and mod-4 counts; per
-//            nonzero wheel byte, its first prime pairs with the previous
-//            byte's last one (gap into the LDS histograms) and the pairs
-//            inside the byte are counted by byte value; gaps that cross
-//            thread runs are joined after a barrier; gaps that cross
-//            windows are joined by stitch_kernel.
+//            order: popcounts give the per-residue and mod-4 counts; one
+//            loop iteration per word, its four wheel bytes unrolled: a
+//            nonzero byte's first prime pairs with the previous byte's last
+//            one (one LDS atomic into the histograms) and the pairs inside
+//            the byte are counted by byte value; gaps that cross thread
+//            runs are joined after a barrier; gaps that cross windows are
+//            joined by stitch_kernel.
+// One wave fetches the next window's index and descriptor during marking,
+// so consecutive windows are not separated by global-memory round trips.
 // The bitmap never leaves LDS.  HBM traffic per window is the descriptor,
 // one 24-byte record, and the (rare) large-gap records; histograms are
 // merged into HBM once per workgroup per job.
@@ -181,6 +184,19 @@ __device__ __forceinline__ void walk_prime(uint32_t* bm, uint32_t p, unsigned lo
     }
 }
 
+// Sum of x over the wave, valid in lane 63: a row-wise inclusive scan
+// (DPP row_shr 1, 2, 4, 8), then the row totals carried by row_bcast 15
+// and 31 (VALU data-parallel moves, no LDS round trips).
+__device__ __forceinline__ uint32_t wave_sum63(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+    return x;
+}
+
 // Thresholds of the merit rule for the even gaps 102..500 (bins 51..250)
 // live in LDS: lgt[h - 51].
 constexpr int kLgtFirst = 51, kLgtCount = 200;
@@ -219,6 +235,29 @@ __device__ __forceinline__ void pair_wide(const PairCtx& c, uint32_t pb, uint32_
                 r.job = c.job;
                 c.large[slot] = r;
             }
+        }
+    }
+}
+
+// The large-gap part of pair_wide for a pair whose histogram count is
+// already taken (g >= c.g_maybe, so g >= 102): a gap beyond the device
+// histogram sets the error bit; a large gap by the reference's rule
+// (g >= 500, or merit >= 10 via the threshold table) becomes a candidate.
+__device__ __forceinline__ void pair_large(const PairCtx& c, uint32_t pb, uint32_t g) {
+    if (g >= 2u * kHistBins) {
+        atomicOr(&c.acc->error, 1ull);
+        return;
+    }
+    const unsigned long long p_before = c.base_num + pb;
+    if (g >= (uint32_t)kLargeAlways || p_before <= c.lgt[(g >> 1) - kLgtFirst]) {
+        const unsigned long long slot = atomicAdd(c.large_n, 1ull);
+        atomicAdd(&c.acc->large_count, 1ull);
+        if (slot < c.large_cap) {
+            LargeRaw r;
+            r.p_before = p_before;
+            r.gap = g;
+            r.job = c.job;
+            c.large[slot] = r;
         }
     }
 }
@@ -307,17 +346,32 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
     // vary (bucketed or not, partial, base-prime counts), and a fixed share
     // per workgroup left a long tail in batches of mixed intervals.  A
     // workgroup's windows still come in increasing order.
+    // The next window's index, descriptor and presieve table offsets are
+    // fetched by one wave during this window's marking (whose work is
+    // pulled from a counter, so the wave that waits for them takes fewer
+    // tasks): the global atomic, the descriptor load and the 64-bit
+    // remainders are off the path from one window to the next.
     __shared__ uint32_t s_next;
+    __shared__ __align__(16) uint32_t s_desc[sizeof(SegDesc) / 4];
+    static_assert(sizeof(SegDesc) % 4 == 0, "descriptor dwords");
+    bool prefetched = false;  // uniform: D of windows after the first is in s_desc
     for (uint32_t d = blockIdx.x; d < a.ndesc;) {
-        const SegDesc D = a.desc[d];
+        SegDesc D;
+        if (prefetched) {
+            uint32_t* dd = reinterpret_cast<uint32_t*>(&D);
+#pragma unroll
+            for (int k = 0; k < (int)(sizeof(SegDesc) / 4); k++)
+                dd[k] = __builtin_amdgcn_readfirstlane(s_desc[k]);
+        } else {
+            D = a.desc[d];
+            if (tid < kPresieveGroups)
+                s_patbase[tid] = (uint32_t)((D.w0 >> 2) % a.pat_len[tid]);
+        }
         if (D.job != cur_job) {
             flush(cur_job);
             cur_job = D.job;
         }
         const uint32_t L4 = D.len >> 2;
-        const uint64_t dw0 = D.w0 >> 2;
-        if (tid < kPresieveGroups)
-            s_patbase[tid] = (uint32_t)(dw0 % a.pat_len[tid]);
         if (tid == 0) {
             s_task = 0;
             s_gmaybe = (uint32_t)kLargeAlways;
@@ -356,6 +410,21 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
         __syncthreads();
 
         // ---- phase 1: base primes 79 <= p, p*p <= window end
+        // (the last wave first fetches what the next window needs: the
+        // presieve above has read s_patbase, and D is in registers)
+        if (tid >= BLOCK - 64) {
+            uint32_t nx = 0;
+            if (lane == 0) nx = gridDim.x + atomicAdd(a.win_next, 1u);
+            nx = __builtin_amdgcn_readfirstlane(nx);
+            if (lane == 0) s_next = nx;
+            if (nx < a.ndesc) {
+                const uint32_t* src = reinterpret_cast<const uint32_t*>(a.desc + nx);
+                if (lane < (int)(sizeof(SegDesc) / 4)) s_desc[lane] = src[lane];
+                const uint64_t nw0 = a.desc[nx].w0;
+                if (lane < kPresieveGroups)
+                    s_patbase[lane] = (uint32_t)((nw0 >> 2) % a.pat_len[lane]);
+            }
+        }
         {
             const uint32_t np = D.nprimes;
             const uint32_t a_end = np < a.tier_a_end ? np : a.tier_a_end;
@@ -497,9 +566,7 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
         // of v[k] is bit k of the number of words with bit j set (a run has
         // at most 32 words, so six planes hold every count)
         uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-        uint32_t nzb = 0;  // nonzero bytes of the run
         auto add_word = [&](uint32_t w) {
-            if constexpr (!EMIT) nzb += (uint32_t)__popc(nonzero_bytes(w));
             uint32_t c = w, t;
             t = v0 & c; v0 ^= c; c = t;
             t = v1 & c; v1 ^= c; c = t;
@@ -597,43 +664,47 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                     a.emit_out[emit_at++] = base_num + prevoff;
                 }
             } else {
-                // one iteration per nonzero byte (30 numbers): its first
-                // prime pairs with the previous byte's last one here; the
-                // pairs inside the byte depend on the byte value alone and
-                // are counted by value (s_bytes), expanded at the flush.
-                // Bytes with primes are ~0.6x the primes at 1e11, and a
-                // wave runs as many iterations as its fullest lane.  The
-                // first prime of the run pairs with itself (gap 0 into the
-                // dropped bin 0).  mq byte address 4*(8*(g/2) + row) =
-                // 16*g + 4*row (g even): equal gaps of different rows fall
-                // in different banks; gaps >= 100 add into the dropped bin
-                // 50 instead of taking a branch around the atomic, and go
-                // to pair_wide.
+                // one pass over the run's words from the first nonzero one,
+                // the four wheel bytes (30 numbers each) of a word unrolled:
+                // a nonzero byte's first prime pairs with the previous
+                // byte's last one here (one LDS atomic: the q=30 row of
+                // p_before for gaps below 100, the main histogram above;
+                // the first prime of the run pairs with itself, gap 0, into
+                // the dropped bin 0); the pairs inside the byte depend on the
+                // byte value alone and are counted by value (s_bytes, only
+                // for bytes with two primes or more), expanded at the flush.
+                // A lane runs one iteration per word (not per byte), so
+                // lanes with more primes do not hold the wave longer.  The
+                // next word is loaded one iteration ahead.  Large-gap
+                // candidates (g >= g_maybe, rare) branch to pair_large.
                 char* const mqb = (char*)mq;
-                uint32_t m = nonzero_bytes(w);
-                for (uint32_t k = 0; k < nzb; k++) {
-                    while (m == 0u) {
-                        i++;
-                        w = bm[i];
-                        m = nonzero_bytes(w);
-                        base += 120u;
+                char* const hb = (char*)hist;
+                uint32_t wn = w;
+                for (; i < iend; i++) {
+                    const uint32_t cw = wn;
+                    wn = bm[min(i + 1u, iend - 1u)];
+                    const uint32_t wb = 120u * i;
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; j++) {
+                        const uint32_t v = (cw >> (8u * j)) & 0xFFu;
+                        if (v) {
+                            const uint32_t blo = (uint32_t)__builtin_ctz(v);
+                            const uint32_t bhi = 31u - (uint32_t)__builtin_clz(v);
+                            const uint32_t bb = wb + 30u * j;
+                            const uint32_t off_lo =
+                                bb + __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, blo | 0x0C0C0C00u);
+                            const uint32_t off_hi =
+                                bb + __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, bhi | 0x0C0C0C00u);
+                            const uint32_t g = off_lo - prevoff;  // even
+                            char* const at = g < 100u ? mqb + (16u * g + rowb)
+                                                      : hb + 2u * min(g, 2u * kHistBins - 2u);
+                            atomicAdd(reinterpret_cast<uint32_t*>(at), 1u);
+                            if (g >= pc.g_maybe) pair_large(pc, prevoff, g);
+                            if (v & (v - 1u)) atomicAdd(&s_bytes[v], 1u);
+                            rowb = bhi << 2;
+                            prevoff = off_hi;
+                        }
                     }
-                    const uint32_t sh = (uint32_t)__builtin_ctz(m) - 7u;  // 8 * byte index
-                    m &= m - 1u;
-                    const uint32_t v = (w >> sh) & 0xFFu;
-                    const uint32_t blo = (uint32_t)__builtin_ctz(v);
-                    const uint32_t bhi = 31u - (uint32_t)__builtin_clz(v);
-                    const uint32_t bb = base + (sh >> 3) * 30u;
-                    const uint32_t off_lo =
-                        bb + __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, blo | 0x0C0C0C00u);
-                    const uint32_t off_hi =
-                        bb + __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, bhi | 0x0C0C0C00u);
-                    const uint32_t g = off_lo - prevoff;
-                    atomicAdd((uint32_t*)(mqb + rowb + 16u * min(g, 100u)), 1u);
-                    if (g >= 100u) pair_wide(pc, prevoff, g);
-                    atomicAdd(&s_bytes[v], 1u);
-                    rowb = bhi << 2;
-                    prevoff = off_hi;
                 }
             }
             prev = prev ? prevoff + 1u : 0u;
@@ -647,10 +718,8 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
                               m30[4], m30[5], m30[6], m30[7]};
 #pragma unroll
             for (int k = 0; k < 10; k++) {
-                uint32_t x = v[k];
-#pragma unroll
-                for (int s = 32; s >= 1; s >>= 1) x += __shfl_xor(x, s);
-                if (lane == 0 && x) atomicAdd(&s_cnt[k], (unsigned long long)x);
+                const uint32_t x = wave_sum63(v[k]);
+                if (lane == 63 && x) atomicAdd(&s_cnt[k], (unsigned long long)x);
             }
         }
         __syncthreads();
@@ -687,9 +756,9 @@ __global__ __launch_bounds__(BLOCK) void sieve_kernel(SieveArgs a) {
             }
             a.rec[d] = rc;
         }
-        if (tid == 0) s_next = gridDim.x + atomicAdd(a.win_next, 1u);
         __syncthreads();
         d = s_next;
+        prefetched = true;
     }
     flush(cur_job);
 }
