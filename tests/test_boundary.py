@@ -78,7 +78,7 @@ def test_driver_missing_binary(tmp_path):
     os.remove(root / "bin" / "voss_v7_live")
     r = subprocess.run([sys.executable, str(root / "src" / "voss_master.py")],
                        capture_output=True, text=True, env=dict(os.environ, VOSS_AMD_HOME=""))
-    assert r.returncode == 1 and "not found" in r.stdout
+    assert r.returncode == 1 and "is not built" in r.stdout
 
 
 VERIFY = os.path.join(PKG, "bin", "verify_cpp")
@@ -93,7 +93,30 @@ def test_verify_cli_input_errors(tmp_path):
     empty.write_text("position,gap,merit\n")
     r = subprocess.run([VERIFY, str(empty), str(tmp_path / "o.csv"), str(tmp_path / "c.csv")],
                        capture_output=True, text=True)
-    assert r.returncode == 1 and "Read 0 gaps" in r.stdout
+    assert r.returncode == 1 and "0 gap rows" in r.stdout
+    bad = tmp_path / "b.csv"
+    bad.write_text("position,gap,merit\nx12,4,0.5\n")
+    r = subprocess.run([VERIFY, str(bad), str(tmp_path / "o.csv"), str(tmp_path / "c.csv")],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "not a gap row" in r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(VERIFY), reason="CLI not built")
+def test_verify_cli_all_cached(tmp_path):
+    """Rows the cache holds are verified without a device: the details file
+    lists every input row in order with verified = 1, merits as %.6f, the
+    ignored lines (empty, no comma) are skipped, the cache gains nothing,
+    and the exit code is 0."""
+    inp = tmp_path / "in.csv"
+    inp.write_text("position,gap,merit\n1327,34,4.7234567\n\nno comma here\n9551,36\n")
+    cache = tmp_path / "cache.csv"
+    cache.write_text("p_after,gap\n9551,36\n1327,34\n")
+    det = tmp_path / "det.csv"
+    r = subprocess.run([VERIFY, str(inp), str(det), str(cache)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert det.read_text() == ("position,gap,merit,verified\n1327,34,4.723457,1\n"
+                               "9551,36,0.000000,1\n")
+    assert cache.read_text() == "p_after,gap\n9551,36\n1327,34\n"
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(PKG, "bin", "voss_v7_live")),

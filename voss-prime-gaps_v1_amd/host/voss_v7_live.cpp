@@ -119,26 +119,63 @@ bool parse_u64(const char* s, uint64_t* v) {
     return true;
 }
 
-void fmt_time(double s, char* buf) {
-    if (s < 60.0) std::snprintf(buf, 32, "%5.1f s", s);
-    else if (s < 3600.0) std::snprintf(buf, 32, "%5.1f m", s / 60.0);
-    else std::snprintf(buf, 32, "%5.1f h", s / 3600.0);
+// A duration as the progress line shows it: seconds below a minute, then
+// minutes below an hour, then hours, one decimal.
+std::string duration_text(double seconds) {
+    static const struct { double limit, unit; char tag; } scales[] = {
+        {60.0, 1.0, 's'}, {3600.0, 60.0, 'm'}, {1e300, 3600.0, 'h'}};
+    char out[32];
+    for (const auto& sc : scales)
+        if (seconds < sc.limit) {
+            std::snprintf(out, sizeof out, "%5.1f %c", seconds / sc.unit, sc.tag);
+            break;
+        }
+    return out;
 }
 
-// print_progress (:317-335)
-void print_progress(int64_t seg, int64_t num_seg, uint64_t total_primes, double elapsed_s,
-                    uint64_t last_prime) {
-    int percent = (int)((seg * 100) / num_seg);
-    double rate = (elapsed_s > 0.1) ? ((double)seg / elapsed_s) : 0.0;
-    double eta_s = (rate > 0.0) ? ((double)(num_seg - seg) / rate) : 0.0;
-    char eta_str[32], el_str[32];
-    fmt_time(eta_s, eta_str);
-    fmt_time(elapsed_s, el_str);
+// The "\r[ %] seg i/n | primes | last p | elapsed | ETA" line the reference
+// program rewrites after every segment group (stdout contract,
+// src/voss_v7_live.cu:317-335); the ETA assumes the rate so far.
+void show_progress(int64_t done, int64_t of, uint64_t primes, uint64_t last, double elapsed) {
+    const double per_s = elapsed > 0.1 ? (double)done / elapsed : 0.0;
+    const double left = per_s > 0.0 ? (double)(of - done) / per_s : 0.0;
     std::fprintf(stdout,
                  "\r[%3d%%] seg %lld/%lld | primes: %-13llu | last p: %-16llu | elapsed: %s | ETA: %s   ",
-                 percent, (long long)seg, (long long)num_seg, (unsigned long long)total_primes,
-                 (unsigned long long)last_prime, el_str, eta_str);
+                 (int)(done * 100 / of), (long long)done, (long long)of, (unsigned long long)primes,
+                 (unsigned long long)last, duration_text(elapsed).c_str(), duration_text(left).c_str());
     std::fflush(stdout);
+}
+
+// Moments of the gap distribution printed after a run: number of gaps,
+// sum of g N(g), mean, standard deviation, skewness and excess kurtosis of
+// g weighted by N(g), in double precision, terms added in increasing g (the
+// printed digits follow from that order of operations).
+struct GapMoments {
+    int64_t count = 0, weighted = 0;
+    double mean = 0, sd = 0, skew = 0, kurt = 0;
+};
+
+GapMoments gap_moments(const int64_t* hist, int bins) {
+    GapMoments m;
+    for (int g = 1; g < bins; g++) {
+        m.count += hist[g];
+        m.weighted += (int64_t)g * hist[g];
+    }
+    m.mean = (double)m.weighted / (double)m.count;
+    double c2 = 0, c3 = 0, c4 = 0;  // central sums: N(g) (g - mean)^k
+    for (int g = 1; g < bins; g++) {
+        if (!hist[g]) continue;
+        const double dev = (double)g - m.mean;
+        const double t = (double)hist[g] * dev;
+        c2 += t * dev;
+        c3 += t * dev * dev;
+        c4 += t * dev * dev * dev;
+    }
+    const double var = c2 / (double)m.count;
+    m.sd = std::sqrt(var);
+    m.skew = c3 / (double)m.count / (var * m.sd);
+    m.kurt = c4 / (double)m.count / (var * var) - 3.0;
+    return m;
 }
 
 }  // namespace
@@ -243,7 +280,7 @@ int main(int argc, char** argv) {
             voss_destroy(ctx);
             return 0;
         }
-        print_progress(end, NUM_SEG, st.ck.total_primes, total_s, st.ck.last_prime);
+        show_progress(end, NUM_SEG, st.ck.total_primes, st.ck.last_prime, total_s);
         seg = end;
     }
     std::printf("\n\n");
@@ -281,56 +318,48 @@ int main(int argc, char** argv) {
     std::printf("Stitch (window edges):             %10.3f s\n", (r->device_ms - r->sieve_ms) / 1000.0);
     std::printf("Host (plan, copies, finalise):     %10.3f s\n\n", r->seconds - r->device_ms / 1000.0);
 
-    // statistics (:652-672)
-    int64_t tgc = 0, wsum = 0;
-    for (int g = 1; g < kMaxGap; g++) { tgc += gaps[g]; wsum += (int64_t)g * gaps[g]; }
-    double mg = (double)wsum / (double)tgc;
-    double vs = 0, ss = 0, ks = 0;
-    for (int g = 1; g < kMaxGap; g++) {
-        if (gaps[g] == 0) continue;
-        double d = (double)g - mg;
-        vs += (double)gaps[g] * d * d;
-        ss += (double)gaps[g] * d * d * d;
-        ks += (double)gaps[g] * d * d * d * d;
-    }
-    double var = vs / (double)tgc, sd = std::sqrt(var);
-    double sk = ss / (double)tgc / (var * sd);
-    double ku = ks / (double)tgc / (var * var) - 3.0;
+    // summary block (stdout contract, src/voss_v7_live.cu:652-705)
+    const GapMoments mo = gap_moments(gaps, kMaxGap);
     std::printf("=== Gap Statistics ===\n");
-    std::printf("Total gaps:  %lld\n", (long long)tgc);
-    std::printf("Mean gap:    %.4f\n", mg);
-    std::printf("Std dev:     %.4f\n", sd);
-    std::printf("Skewness:    %.4f\n", sk);
-    std::printf("Kurtosis:    %.4f\n\n", ku);
+    std::printf("Total gaps:  %lld\n", (long long)mo.count);
+    std::printf("Mean gap:    %.4f\n", mo.mean);
+    std::printf("Std dev:     %.4f\n", mo.sd);
+    std::printf("Skewness:    %.4f\n", mo.skew);
+    std::printf("Kurtosis:    %.4f\n\n", mo.kurt);
     std::printf("=== Gap Counts ===\n");
     for (int g : {1, 2, 4, 6, 12, 30})
         std::printf("Gap %d:%s%lld\n", g, g < 10 ? "  " : " ", (long long)gaps[g]);
     std::printf("\n");
 
-    const unsigned long long p41 = r->c1 + 1, p43 = r->c3 + 1;  // :682
-    const long long diff = (long long)p43 - (long long)p41;
+    // pi(x;4,1) and pi(x;4,3) count 5 and 3 besides the wheel primes
+    const unsigned long long pi41 = r->c1 + 1, pi43 = r->c3 + 1;
+    const long long bias = (long long)pi43 - (long long)pi41;
     std::printf("=== Chebyshev Bias ===\n");
-    std::printf("pi(x;4,1) = %llu\n", p41);
-    std::printf("pi(x;4,3) = %llu\n", p43);
-    std::printf("Diff      = %lld\n\n", diff);
-    const double p6p2 = (double)gaps[6] / (double)gaps[2];
+    std::printf("pi(x;4,1) = %llu\n", pi41);
+    std::printf("pi(x;4,3) = %llu\n", pi43);
+    std::printf("Diff      = %lld\n\n", bias);
+    const double p6p2 = (double)gaps[6] / (double)gaps[2];  // also the hl_trend row
     std::printf("=== Gap Repulsion ===\n");
     std::printf("P(6)/P(2) = %.6f\n\n", p6p2);
 
-    const uint64_t total = r->total_primes;
+    // the three identities every complete run satisfies
+    const uint64_t pi = r->total_primes;
+    const bool ok1 = mo.count == (int64_t)(pi - 1);
+    const bool ok2 = mo.weighted == (int64_t)(r->last_prime - 2);
+    const bool ok3 = pi41 + pi43 + 1 == pi;
     std::printf("=== Self-Consistency Checks ===\n");
     std::printf("[Check 1] sum(N(g)) = pi(N) - 1\n");
-    std::printf("  sum(N(g))  = %lld\n", (long long)tgc);
-    std::printf("  pi(N) - 1  = %lld\n", (long long)(total - 1));
-    std::printf("  -> %s\n", (tgc == (int64_t)(total - 1)) ? "PASS" : "FAIL");
+    std::printf("  sum(N(g))  = %lld\n", (long long)mo.count);
+    std::printf("  pi(N) - 1  = %lld\n", (long long)(pi - 1));
+    std::printf("  -> %s\n", ok1 ? "PASS" : "FAIL");
     std::printf("[Check 2] sum(g*N(g)) = p_last - 2\n");
-    std::printf("  sum(g*N(g)) = %lld\n", (long long)wsum);
+    std::printf("  sum(g*N(g)) = %lld\n", (long long)mo.weighted);
     std::printf("  p_last - 2  = %lld\n", (long long)(r->last_prime - 2));
-    std::printf("  -> %s\n", (wsum == (int64_t)(r->last_prime - 2)) ? "PASS" : "FAIL");
+    std::printf("  -> %s\n", ok2 ? "PASS" : "FAIL");
     std::printf("[Check 3] pi(4,1) + pi(4,3) + 1 = pi(N)\n");
-    std::printf("  pi41+pi43+1 = %llu\n", p41 + p43 + 1);
-    std::printf("  pi(N)       = %llu\n", (unsigned long long)total);
-    std::printf("  -> %s\n\n", ((p41 + p43 + 1) == total) ? "PASS" : "FAIL");
+    std::printf("  pi41+pi43+1 = %llu\n", pi41 + pi43 + 1);
+    std::printf("  pi(N)       = %llu\n", (unsigned long long)pi);
+    std::printf("  -> %s\n\n", ok3 ? "PASS" : "FAIL");
 
     // ---- files (formats of :707-827)
     FILE* fp = std::fopen("gap_histogram.csv", "w");
@@ -383,7 +412,7 @@ int main(int argc, char** argv) {
     fp = std::fopen("chebyshev.csv", "w");
     if (fp) {
         std::fprintf(fp, "N,pi_4_1,pi_4_3,difference\n");
-        std::fprintf(fp, "%lld,%llu,%llu,%lld\n", (long long)N, p41, p43, diff);
+        std::fprintf(fp, "%lld,%llu,%llu,%lld\n", (long long)N, pi41, pi43, bias);
         std::fclose(fp);
         std::printf("OK chebyshev.csv\n");
     }

@@ -1,108 +1,117 @@
 N_VALUE = 100000000000      # 10^11 -- run_voss.py rewrites the first N_VALUE
 SEG_NUM = 30000000000       # 3x10^10 -- and the first SEG_NUM assignment
-# ============================================================
-# VOSS master, MI355X drop-in.  Replaces the reference's src/voss_master.py:
-# instead of patching constants into a CUDA source and compiling it with
-# nvcc (src/voss_master.py:64-85, :138-143), it runs the prebuilt gfx950
-# program voss_v7_live with N and SEG_NUM as arguments, cwd = results/,
-# streams its output and checks "Total primes:" against the OEIS table in
-# data/verification_db.json (:150-179).  The two assignments above must stay
-# the first lines that match run_voss.py's regexes (run_voss.py:118-121).
-# ============================================================
+"""VOSS driver for the MI355X build (drop-in for the reference's
+src/voss_master.py).
+
+run_voss.py (run_voss.py:111-123, :198) rewrites the first `N_VALUE = <n>` and
+`SEG_NUM = <n>` assignments of this file, runs it with the repository root as
+working directory, streams its output and stops the pipeline on a non-zero
+exit code; afterwards it reads results/large_gaps.csv.  Those two
+assignments therefore stay the first lines of the file.
+
+What this driver does: it runs the prebuilt gfx950 program voss_v7_live with
+N and SEG_NUM as arguments (nothing is patched or compiled), with results/ as
+its working directory, where the program writes its CSV files; it passes the
+program's output through as it arrives (the progress line rewrites itself
+with carriage returns); it compares the program's "Total primes: <pi>" with
+pi(10^k) from data/verification_db.json when N is a power of ten (no
+download is attempted); and it exits with the program's exit code, so a
+failed run stops run_voss.py instead of leaving stale result files behind.
+"""
 import json
-import math
 import os
 import re
-import shutil
 import subprocess
 import sys
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-ROOT = os.path.abspath(os.path.join(HERE, '..'))
-RESULTS = os.path.join(ROOT, 'results')
-DB_FILE = os.path.join(ROOT, 'data', 'verification_db.json')
+REPO = os.path.abspath(os.path.join(os.path.dirname(os.path.abspath(__file__)), ".."))
+OUT_DIR = os.path.join(REPO, "results")
+PI_TABLE = os.path.join(REPO, "data", "verification_db.json")
+PROGRAM = "voss_v7_live"
 
 
-def find_binary(name):
-    """The program built by voss-prime-gaps_v1_amd/build.py: from
-    $VOSS_AMD_HOME, from this tree's bin/, or beside this package."""
-    cands = []
-    home = os.environ.get('VOSS_AMD_HOME')
+def program_path():
+    """voss_v7_live as built by voss-prime-gaps_v1_amd/build.py: under
+    $VOSS_AMD_HOME (the MI355X package or its bin/), else this tree's bin/."""
+    places = []
+    home = os.environ.get("VOSS_AMD_HOME")
     if home:
-        cands += [os.path.join(home, 'voss-prime-gaps_v1_amd', 'bin', name),
-                  os.path.join(home, 'bin', name)]
-    cands += [os.path.join(ROOT, 'bin', name)]
-    for c in cands:
-        if os.path.isfile(c) and os.access(c, os.X_OK):
-            return c
-    return None
+        places.append(os.path.join(home, "voss-prime-gaps_v1_amd", "bin", PROGRAM))
+        places.append(os.path.join(home, "bin", PROGRAM))
+    places.append(os.path.join(REPO, "bin", PROGRAM))
+    return next((p for p in places if os.path.isfile(p) and os.access(p, os.X_OK)), None)
 
 
-def load_db():
-    if not os.path.exists(DB_FILE):
-        print('No ' + DB_FILE + ' (no download is attempted); OEIS check skipped')
-        return {}
-    with open(DB_FILE) as f:
-        db = json.load(f)
-    print('DB OK: ' + DB_FILE)
-    return db.get('pi_of_10_pow_n', {})
+def power_of_ten(n):
+    """k when n == 10**k, else None."""
+    k, v = 0, 1
+    while v < n:
+        v *= 10
+        k += 1
+    return k if v == n else None
+
+
+def published_pi(n):
+    """pi(n) from the local OEIS A006880 table when n is a power of ten."""
+    k = power_of_ten(n)
+    if k is None or not os.path.isfile(PI_TABLE):
+        return None
+    with open(PI_TABLE) as f:
+        table = json.load(f).get("pi_of_10_pow_n", {})
+    v = table.get(str(k))
+    return int(v) if v is not None else None
+
+
+def run_streamed(argv, cwd):
+    """Runs argv in cwd, copying its merged stdout/stderr to ours as it
+    comes; returns (exit code, everything it printed)."""
+    seen = []
+    child = subprocess.Popen(argv, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    try:
+        while True:
+            chunk = os.read(child.stdout.fileno(), 4096)
+            if not chunk:
+                break
+            seen.append(chunk)
+            sys.stdout.buffer.write(chunk)
+            sys.stdout.flush()
+    except KeyboardInterrupt:
+        child.terminate()
+    code = child.wait()
+    return code, b"".join(seen).decode("utf-8", errors="replace")
 
 
 def main():
-    print('=' * 60)
-    print('VOSS Master (MI355X)')
-    print('=' * 60)
-    print(f'ROOT  = {ROOT}')
-    binary = find_binary('voss_v7_live')
-    if binary is None:
-        print('ERROR: voss_v7_live not found (build voss-prime-gaps_v1_amd and set VOSS_AMD_HOME)')
-        sys.exit(1)
-    print(f'Binary = {binary}')
-    print()
-    pi_db = load_db()
-    n_log10 = round(math.log10(N_VALUE))
-    is_pow10 = abs(10**n_log10 - N_VALUE) / N_VALUE < 1e-9
-    expected_pi = pi_db.get(str(n_log10)) if is_pow10 else None
-    if expected_pi:
-        print(f'pi(10^{n_log10}) from OEIS = {expected_pi:,}')
-    print()
-    print(f'N={N_VALUE:,}  SEG_NUM={SEG_NUM:,}  NUM_SEG={math.ceil(N_VALUE / SEG_NUM)}')
-    print('=' * 60)
-    os.makedirs(RESULTS, exist_ok=True)
-    print(f'Running in: {RESULTS}\n')
-    proc = subprocess.Popen([binary, str(N_VALUE), str(SEG_NUM)], cwd=RESULTS,
-                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT, bufsize=0)
-    captured = []
-    try:
-        while True:
-            ch = proc.stdout.read(1)
-            if not ch:
-                break
-            t = ch.decode('utf-8', errors='replace')
-            captured.append(t)
-            sys.stdout.write(t)
-            sys.stdout.flush()
-    except KeyboardInterrupt:
-        proc.terminate()
-    rc = proc.wait()
-    full = ''.join(captured)
-    print('\n' + '=' * 60)
-    print('=== AUTO-VERIFICATION vs OEIS A006880 ===')
-    print('=' * 60)
-    m = re.search(r'Total primes:\s*(\d+)', full)
-    actual = int(m.group(1)) if m else None
-    if actual and expected_pi:
-        print(f'OEIS pi(10^{n_log10}) = {expected_pi:>25,}')
-        print(f'VOSS pi(10^{n_log10}) = {actual:>25,}')
-        d = actual - expected_pi
-        print(f'Diff = {d:+,}')
-        print('PASS' if d == 0 else f'FAIL ({d:,})')
+    n, seg = int(N_VALUE), int(SEG_NUM)
+    exe = program_path()
+    bar = "-" * 60
+    print(bar)
+    print(f"VOSS driver (MI355X build)   N = {n:,}   SEG_NUM = {seg:,}   "
+          f"logical segments = {-(-n // seg)}")
+    print(bar)
+    if exe is None:
+        print(f"{PROGRAM} is not built: run voss-prime-gaps_v1_amd/build.py and point "
+              "VOSS_AMD_HOME at it")
+        return 1
+    want = published_pi(n)
+    os.makedirs(OUT_DIR, exist_ok=True)
+    print(f"program  {exe}\noutputs  {OUT_DIR}\n", flush=True)
+    code, text = run_streamed([exe, str(n), str(seg)], OUT_DIR)
+    hit = re.search(r"Total primes:\s*(\d+)", text)
+    got = int(hit.group(1)) if hit else None
+    print("\n" + bar)
+    if got is not None and want is not None:
+        verdict = "PASS" if got == want else f"FAIL (off by {got - want:+,})"
+        print(f"pi({n:,}) = {got:,}; OEIS A006880 gives {want:,}: {verdict}")
+    elif got is not None:
+        print(f"pi({n:,}) = {got:,} (no published value to compare with)")
     else:
-        print('Verification skipped (not a power of 10, or parse failed).')
-    if rc != 0:
-        print(f'voss_v7_live exited with code {rc}')
-        sys.exit(rc)
+        print("the program printed no 'Total primes:' line")
+    if code != 0:
+        print(f"{PROGRAM} exited with code {code}")
+    print(bar)
+    return code
 
 
-if __name__ == '__main__':
-    main()
+if __name__ == "__main__":
+    sys.exit(main())
