@@ -15,8 +15,10 @@ read-modify-write: one per (prime, word) for the presieved primes below 32
 and one per mark for the rest, plus the bitmap written once and read once)
 over its average duration from HIP events on the library's stream.
 `cpu_baseline` times the CPU oracle (a port of the reference semantics) on
-the whole workload, [1, 1e11], on the host cores (about 6-10 s on 16
-threads).
+the whole workload, [1, 1e11], on every host CPU the process may use: the
+median of 5 runs after a warm-up (about 20 s in all), with config 1's
+[2, 1e9] run and its large-gap verification beside it and the host's CPU
+model, clock and counts recorded.
 """
 import argparse
 import json
@@ -44,7 +46,7 @@ LDS_PEAK_GBS = 128 * 256 * 2.4e9 / 1e9
 # runs (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, scaled by a 1 GiB copy kernel);
 # the commands and raw counters are in
 # profiles/sieve_kernel_plain_1e11_hbm_traffic.md.
-SIEVE_1E11_HBM_BYTES = (2 * 3384.8 + 18617.8) * 1024
+SIEVE_1E11_HBM_BYTES = (2 * 3448.5 + 18599.3) * 1024
 
 
 def model_lds_bytes(N):
@@ -172,16 +174,64 @@ def mr_mulmods(n):
     return work
 
 
+def host_cpu():
+    """The host the CPU baselines run on: logical CPUs (nproc), the CPUs this
+    process may use (its affinity set, the share of the host it was given),
+    the model and clock from /proc/cpuinfo, OMP_NUM_THREADS."""
+    info = {"nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        with open("/proc/cpuinfo") as f:
+            lines = f.read().splitlines()
+        model = [l.split(":", 1)[1].strip() for l in lines if l.startswith("model name")]
+        mhz = [float(l.split(":", 1)[1]) for l in lines if l.startswith("cpu MHz")]
+        info["model"] = model[0] if model else None
+        info["mhz"] = round(statistics.median(mhz), 1) if mhz else None
+    except OSError:
+        pass
+    return info
+
+
+def cpu_threads():
+    """Threads for the CPU baselines: every CPU this process may use (all
+    host cores it was given), or OMP_NUM_THREADS when that is set lower."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and 0 < int(omp) < n:
+        n = int(omp)
+    return n
+
+
+def median_of(fn, runs=5, warmup=1):
+    """fn() timed runs times after warmup untimed calls: (median s, all s, last result)."""
+    out = None
+    for _ in range(warmup):
+        out = fn()
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t)
+    return statistics.median(times), times, out
+
+
 def cpu_baseline(threads):
+    """The oracle (the reference's plain-mode semantics restated in C with
+    OpenMP; the reference has no CPU sieve) on the whole workload [1, 1e11],
+    and on config 1, [2, 1e9]: median of 5 runs after one warm-up."""
     from oracle import oracle as O
-    t = time.perf_counter()
-    r = O.run_plain(CPU_SAMPLE_N, SEG_NUM, threads=threads)
-    dt = time.perf_counter() - t
+    med, times, r = median_of(lambda: O.run_plain(CPU_SAMPLE_N, SEG_NUM, threads=threads,
+                                                  large_cap=16))
     assert r["total_primes"] == 4118054813  # pi(1e11)
-    return {"value": CPU_SAMPLE_N / dt, "unit": "numbers/s", "cores": threads,
-            "kind": "port", "seconds": round(dt, 3),
+    med9, times9, r9 = median_of(lambda: O.run_plain(10**9, SEG_NUM, threads=threads))
+    assert r9["total_primes"] == 50847534  # pi(1e9)
+    return {"value": CPU_SAMPLE_N / med, "unit": "numbers/s", "cores": threads,
+            "kind": "port", "seconds": round(med, 3), "runs_s": [round(t, 3) for t in times],
             "sample": f"the whole workload, plain [1, {CPU_SAMPLE_N:.0e}] SEG_NUM=3e10, "
-                      f"oracle C/OpenMP, {threads} threads"}
+                      f"oracle C/OpenMP, {threads} threads, median of 5 after a warm-up",
+            "config1_1e9": {"numbers_per_s": 10**9 / med9, "seconds": round(med9, 4),
+                            "runs_s": [round(t, 4) for t in times9]},
+            "host": host_cpu(), "large_1e9": r9["large"]}
 
 
 def batch_leg(eng, W, with_plain=True, autotune=True, mr_count=10**7):
@@ -291,15 +341,27 @@ def verify_leg(eng, large, no_cpu):
     if no_cpu:
         return out
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    t = time.perf_counter()
-    ref = O.verify_gaps(pa, g, threads=threads)
-    dt = time.perf_counter() - t
+    threads = cpu_threads()
+    dt, _, ref = median_of(lambda: O.verify_gaps(pa, g, threads=threads))
     assert np.array_equal(ref, ok)
     out["cpu_baseline"] = {"value": len(pa) / dt, "unit": "gaps/s", "cores": threads,
                            "kind": "port", "sample": "the whole list, verify_gap restated "
-                           "with __uint128_t (oracle, OpenMP)"}
+                           "with __uint128_t (oracle, OpenMP), median of 5 after a warm-up"}
     return out
+
+
+def verify_config1(eng, large, threads):
+    """verify_cpp semantics on config 1's large-gap list ([2, 1e9]): the
+    oracle's verify_gap on the host cores and voss_verify_gaps on the GPU,
+    each the median of 5 after a warm-up."""
+    from oracle import oracle as O
+    pa = np.array([p for p, _, _ in large], dtype=np.uint64)
+    g = np.array([x for _, x, _ in large], dtype=np.int32)
+    cpu_s, _, ref = median_of(lambda: O.verify_gaps(pa, g, threads=threads))
+    gpu_s, _, got = median_of(lambda: eng.verify_gaps(pa, g))
+    assert ref.all() and np.array_equal(ref, got)
+    return {"gaps": int(len(pa)), "cpu_gaps_per_s": len(pa) / cpu_s, "cpu_seconds": cpu_s,
+            "gpu_gaps_per_s": len(pa) / gpu_s, "gpu_seconds": gpu_s, "cores": threads}
 
 
 def batch_with_mr(eng, lo, hi, cand, rb, same, replay_s, out):
@@ -334,11 +396,15 @@ def batch_with_mr(eng, lo, hi, cand, rb, same, replay_s, out):
             "bit_exact": bool(same(rb, rr)), "verdicts_equal": bool(np.array_equal(v, verdict))}
 
 
-def serve_leg(eng, W, n_req=300, n_cal=80):
+def serve_leg(eng, W, n_req=300, n_cal=80, loads=(0.7, 1.3)):
     """Config 5: a Poisson stream (half interval requests with start in
     [1e10, 1e16) and length log-uniform in [1e5, 1e10], half MR batches of
-    1e3..1e6 candidates) at 0.7x the capacity of the plain path, which runs
-    one request at a time in arrival order.  Latency = completion - arrival."""
+    1e3..1e6 candidates), offered at each of `loads` times the capacity of
+    the plain path (which runs one request at a time in arrival order, as
+    fast as it can): below capacity both paths keep up and latency shows
+    the difference; above it the plain path's queue grows while continuous
+    batching overlaps requests on the device.  Latency = completion -
+    arrival; req/s = requests / (last completion - first arrival)."""
     import threading
     reqs = W.serve_requests(n_req + n_cal, 1.0)
     rng = np.random.default_rng(W.SEED0 + 4)
@@ -358,51 +424,53 @@ def serve_leg(eng, W, n_req=300, n_cal=80):
     for k in range(n_req, n_req + n_cal):  # calibration requests (not in the stream)
         run_alone(k)
     cap = n_cal / (time.perf_counter() - t)
-    lam = 0.7 * cap
-    arrivals = [a / lam for a, _, _ in reqs[:n_req]]  # the stream at rate lam
-
-    # plain path: one request at a time, FIFO
-    lat_plain = []
-    t0 = time.perf_counter()
-    for k in range(n_req):
-        now = time.perf_counter() - t0
-        if now < arrivals[k]:
-            time.sleep(arrivals[k] - now)
-        run_alone(k)
-        lat_plain.append(time.perf_counter() - t0 - arrivals[k])
-    span_plain = time.perf_counter() - t0
-
-    # continuous batching
-    ids = [None] * n_req
-    with eng.serve(max_active=64, windows_per_step=4096, large_capacity=64) as srv:
-        t0 = time.perf_counter()
-
-        def submitter():
-            for k in range(n_req):
-                now = time.perf_counter() - t0
-                if now < arrivals[k]:
-                    time.sleep(arrivals[k] - now)
-                _, kind, p = reqs[k]
-                ids[k] = srv.submit_interval(*p) if kind == "interval" else srv.submit_mr(payload[k])
-
-        th = threading.Thread(target=submitter)
-        th.start()
-        th.join()
-        lat_srv = [srv.wait(rid)[1] for rid in ids]
-        span_srv = time.perf_counter() - t0
-
-    numbers = sum(p[1] - p[0] for _, kind, p in reqs[:n_req] if kind == "interval")
-    tests = sum(p for _, kind, p in reqs[:n_req] if kind == "mr")
 
     def pct(x, q):
         return float(np.percentile(np.array(x) * 1e3, q))
 
-    return {"requests": n_req, "rate_req_per_s": lam, "plain_capacity_req_per_s": cap,
-            "interval_numbers": numbers, "mr_tests": tests,
-            "served": {"p50_ms": pct(lat_srv, 50), "p99_ms": pct(lat_srv, 99),
-                       "makespan_s": span_srv, "req_per_s": n_req / span_srv},
-            "plain": {"p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
-                      "makespan_s": span_plain, "req_per_s": n_req / span_plain}}
+    def summary(lat, span):
+        return {"p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "makespan_s": span,
+                "req_per_s": n_req / span}
+
+    out = {"requests": n_req, "plain_capacity_req_per_s": cap,
+           "interval_numbers": sum(p[1] - p[0] for _, kind, p in reqs[:n_req] if kind == "interval"),
+           "mr_tests": sum(p for _, kind, p in reqs[:n_req] if kind == "mr"), "loads": {}}
+    for load in loads:
+        lam = load * cap
+        arrivals = [a / lam for a, _, _ in reqs[:n_req]]  # the stream at rate lam
+        # plain path: one request at a time, FIFO
+        lat_plain = []
+        t0 = time.perf_counter()
+        for k in range(n_req):
+            now = time.perf_counter() - t0
+            if now < arrivals[k]:
+                time.sleep(arrivals[k] - now)
+            run_alone(k)
+            lat_plain.append(time.perf_counter() - t0 - arrivals[k])
+        span_plain = time.perf_counter() - t0
+        # continuous batching
+        ids = [None] * n_req
+        with eng.serve(max_active=64, windows_per_step=4096, large_capacity=64) as srv:
+            t0 = time.perf_counter()
+
+            def submitter():
+                for k in range(n_req):
+                    now = time.perf_counter() - t0
+                    if now < arrivals[k]:
+                        time.sleep(arrivals[k] - now)
+                    _, kind, p = reqs[k]
+                    ids[k] = srv.submit_interval(*p) if kind == "interval" else \
+                        srv.submit_mr(payload[k])
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            th.join()
+            lat_srv = [srv.wait(rid)[1] for rid in ids]
+            span_srv = time.perf_counter() - t0
+        out["loads"][f"{load:g}x"] = {"rate_req_per_s": lam,
+                                      "served": summary(lat_srv, span_srv),
+                                      "plain": summary(lat_plain, span_plain)}
+    return out
 
 
 def interval_leg(eng):
@@ -527,15 +595,13 @@ def main():
                           "sample": f"exact count on the first {n_sample} candidates"}
     if not args.no_cpu:
         from oracle import oracle as O
-        threads = min(16, os.cpu_count() or 1)
-        sample = cand[: min(len(cand), 2 * 10**6)]
-        t = time.perf_counter()
-        ref = O.is_prime_batch(sample, threads=threads)
-        dt = time.perf_counter() - t
-        assert np.array_equal(ref, verdict[: len(sample)])
-        mr["cpu_baseline"] = {"value": len(sample) / dt, "unit": "tests/s", "cores": threads,
-                              "kind": "port", "sample": f"first {len(sample)} candidates, "
-                              "is_prime_u64 restated with __uint128_t (oracle, OpenMP)"}
+        threads = cpu_threads()
+        dt, _, ref = median_of(lambda: O.is_prime_batch(cand, threads=threads))
+        assert np.array_equal(ref, verdict)
+        mr["cpu_baseline"] = {"value": len(cand) / dt, "unit": "tests/s", "cores": threads,
+                              "kind": "port", "sample": f"all {len(cand)} candidates, "
+                              "is_prime_u64 restated with __uint128_t (oracle, OpenMP), "
+                              "median of 5 after a warm-up"}
 
     out = {
         "metric": "sieved numbers/s at N=1e11 (plain [1,N], SEG_NUM=3e10, bit-exact outputs)",
@@ -575,8 +641,10 @@ def main():
         "legs": legs,
     }
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(threads)
+        threads = cpu_threads()
+        cb = cpu_baseline(threads)
+        cb["config1_verify"] = verify_config1(eng, cb.pop("large_1e9"), threads)
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
     eng.close()
 
