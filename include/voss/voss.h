@@ -275,13 +275,15 @@ int voss_verify_gaps(voss_ctx* ctx, const uint64_t* p_after,
 
 /* Continuous-batching server (no reference counterpart: the reference runs
  * one range per process).  A scheduler thread owns the context while the
- * server runs.  Every step it admits queued requests while request slots
- * and the HBM budget allow, then runs one sieve round made of up to
- * windows_per_step LDS windows taken round-robin from all active interval
- * requests (their accumulators stay in HBM slots across steps), and one
- * Miller-Rabin launch over the admitted verify batches on a second stream,
- * overlapping the sieve.  Results are exactly those of voss_run_batch /
- * voss_mr_batch on the same request. */
+ * server runs.  It admits queued requests while request slots and the HBM
+ * budget allow.  Interval requests keep their accumulators in HBM slots
+ * until they complete; each sieve step is made of up to windows_per_step
+ * LDS windows taken round-robin from all active interval requests, and
+ * steps are enqueued back to back (the host prepares the next step while
+ * the device runs the current one, and collects a step's results after its
+ * event).  Verify batches run on a second stream beside the sieve steps.
+ * Results are exactly those of voss_run_batch / voss_mr_batch on the same
+ * request. */
 typedef struct voss_server voss_server;
 
 typedef struct {
@@ -302,6 +304,12 @@ int voss_serve_poll(voss_server* s, uint64_t id, int* done, voss_interval_result
                     uint8_t* verdict, double* latency_s);
 int voss_serve_wait(voss_server* s, uint64_t id, voss_interval_result* out,
                     uint8_t* verdict, double* latency_s);
+/* Counters of a running server (no reference counterpart): sieve steps
+ * and LDS windows enqueued so far, the device time of the steps collected so
+ * far (HIP events around each step), and the wall time since
+ * voss_serve_start.  Any pointer may be NULL. */
+int voss_serve_stats(voss_server* s, uint64_t* steps, uint64_t* windows, double* device_busy_s,
+                     double* wall_s);
 /* Finishes every submitted request, stops the scheduler, frees the server. */
 int voss_serve_stop(voss_server* s);
 

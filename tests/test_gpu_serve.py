@@ -65,3 +65,42 @@ def test_served_equals_alone(engine, cfg):
     for k in range(len(mr)):
         v, _ = got[("mr", k)]
         assert np.array_equal(v, alone_mr[k])
+
+
+def test_served_step_counters(engine):
+    """The server's counters: every window of every interval request is
+    enqueued exactly once (no restarts here), in at least as many steps as
+    the step size requires, and the steps' device time is within the wall
+    time."""
+    iv, _ = workload()
+    alone_windows = sum(engine.run_batch([a], [b])[1]["windows"] for a, b in iv)
+    with engine.serve(max_active=8, windows_per_step=64, large_capacity=64) as srv:
+        ids = [srv.submit_interval(a, b) for a, b in iv]
+        for rid in ids:
+            srv.wait(rid)
+        st = srv.stats()
+    assert st["windows"] == alone_windows
+    assert st["steps"] >= alone_windows / 64
+    assert 0 < st["device_busy_s"] <= st["wall_s"]
+
+
+def test_served_bucket_overflow_restarts_exactly(engine):
+    """Bucket rooms at 50 % of their estimate: the first steps with bucketed
+    windows overflow, their requests start over with exact rooms, and every
+    result still equals the request run alone."""
+    iv = [(10**15 + k * 10**9, 10**15 + k * 10**9 + 5 * 10**7) for k in range(4)]
+    iv += [(10**16 - 10**8, 10**16 - 10**8 + 3 * 10**7), (10**12, 10**12 + 10**6)]
+    alone = [engine.run_batch([a], [b], large_cap=64)[0][0] for a, b in iv]
+    t0 = engine.tuning()
+    engine.set_tuning(bucket_room_pct=50, bucket_limit=1 << 17, bucket_min_windows=2)
+    try:
+        print("[gpu] serve with undersized bucket rooms", flush=True)
+        with engine.serve(max_active=4, windows_per_step=16, large_capacity=64) as srv:
+            got = [srv.wait(srv.submit_interval(a, b))[0] for a, b in iv]
+            st = srv.stats()
+    finally:
+        engine.set_tuning(**t0)
+    for g, a in zip(got, alone):
+        same(g, a)
+    planned = sum(engine.run_batch([a], [b])[1]["windows"] for a, b in iv)
+    assert st["windows"] > planned  # some windows ran twice (the restarts)

@@ -1,31 +1,10 @@
 // api.hip -- the C ABI of include/voss/voss.h: context, planning of LDS
 // windows, kernel launches and the host finalisation that turns the device
-// accumulators into the reference's output arrays.
-#include <hip/hip_runtime.h>
+// accumulators into the reference's output arrays.  Serving is in serve.hip.
+#include "engine.h"
 
-#include <algorithm>
-#include <atomic>
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <condition_variable>
-#include <cstring>
-#include <functional>
-#include <memory>
-#include <mutex>
-#include <string>
-#include <thread>
-#include <vector>
-
-#include "call_gate.h"
-#include "mr_launch.h"
-#include "sieve_launch.h"
-#include "voss/voss.h"
-#include "wheel.h"
-
-using namespace voss;
-
-namespace {
+namespace voss {
+namespace core {
 
 thread_local std::string g_err;
 
@@ -37,90 +16,6 @@ int fail(int code, const std::string& msg) {
 int hip_fail(hipError_t e, const char* what) {
     return fail(VOSS_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
-
-#define HIPCHK(call)                                         \
-    do {                                                     \
-        hipError_t _e = (call);                              \
-        if (_e != hipSuccess) return hip_fail(_e, #call);    \
-    } while (0)
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t cap = 0;  // elements
-    hipError_t reserve(size_t n) {
-        if (n <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        size_t want = std::max<size_t>(n, 64);
-        hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
-        if (e == hipSuccess) cap = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-// Grow-only pinned host staging buffer (hipHostMalloc).
-template <typename T>
-struct HostBuf {
-    T* p = nullptr;
-    size_t cap = 0;
-    hipError_t reserve(size_t n) {
-        if (n <= cap) return hipSuccess;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-        const size_t want = std::max<size_t>(n, 64);
-        hipError_t e = hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault);
-        if (e == hipSuccess) cap = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-// Buffers of one execution of a plan, device and pinned host.  The context
-// owns one set for its calls; a prepared batch owns its own, so its
-// captured graph keeps valid pointers between replays.
-struct ExecBufs {
-    DevBuf<SegDesc> d_desc;
-    DevBuf<SegRec> d_rec;
-    DevBuf<JobAcc> d_acc;
-    DevBuf<LargeRaw> d_large;
-    DevBuf<unsigned long long> d_flags;  // [0] large-gap records, [1] bucket overflow,
-                                         // [2] bucket task counter, [3] sieve
-                                         // window counter (low 32 bits)
-    DevBuf<BucketTask> d_btask;
-    DevBuf<uint32_t> d_bcnt;
-    DevBuf<uint32_t> d_bcur;
-    DevBuf<unsigned long long> d_boff;
-    DevBuf<uint32_t> d_bent;
-    HostBuf<SegDesc> h_desc;
-    HostBuf<BucketTask> h_btask;
-    HostBuf<unsigned long long> h_boff;
-    HostBuf<uint32_t> h_bcnt;
-    HostBuf<SegRec> h_rec;
-    HostBuf<JobAcc> h_acc;
-    HostBuf<LargeRaw> h_large;
-    HostBuf<unsigned long long> h_flags;
-    void release() {
-        d_desc.release(); d_rec.release(); d_acc.release(); d_large.release();
-        d_flags.release(); d_btask.release(); d_bcnt.release(); d_bcur.release();
-        d_boff.release(); d_bent.release(); h_desc.release(); h_btask.release();
-        h_boff.release(); h_bcnt.release(); h_rec.release(); h_acc.release();
-        h_large.release(); h_flags.release();
-    }
-};
-
-constexpr uint32_t kW30Mask[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 
 uint64_t isqrt_u64(uint64_t n) {
     uint64_t r = (uint64_t)std::sqrt((long double)n);
@@ -171,138 +66,6 @@ bool merit_ge10(uint64_t p_before, uint32_t g) {
     return (double)g / std::log((double)p_before) >= 10.0;
 }
 
-// A fixed set of host worker threads, started on first use, for the host
-// steps of a run (planning, accumulation, merits): starting threads per
-// call cost more than the sub-millisecond steps of a small run.  One
-// caller at a time; a second concurrent caller, or a call from inside a
-// worker, runs its work inline.
-class HostPool {
-public:
-    static HostPool& get() {
-        static HostPool pool;
-        return pool;
-    }
-    size_t workers() const { return th_.size(); }
-    // job(t) for t in [0, n), the calling thread taking part; false if the
-    // pool is busy (the caller then does the work itself)
-    bool run(size_t n, const std::function<void(size_t)>& job) {
-        if (in_worker() || !busy_.try_lock()) return false;
-        {
-            std::lock_guard<std::mutex> g(m_);
-            job_ = &job;
-            n_ = n;
-            next_ = 0;
-            done_ = 0;
-            gen_++;
-        }
-        cv_.notify_all();
-        in_worker() = true;  // a nested call from one of our jobs runs inline
-        work();
-        in_worker() = false;
-        {
-            std::unique_lock<std::mutex> g(m_);
-            done_cv_.wait(g, [&] { return done_ == n_; });
-            job_ = nullptr;
-        }
-        busy_.unlock();
-        return true;
-    }
-
-private:
-    HostPool() {
-        const size_t hw = std::max<unsigned>(1u, std::thread::hardware_concurrency());
-        const size_t nw = std::min<size_t>(15, hw > 1 ? hw - 1 : 0);
-        for (size_t i = 0; i < nw; i++) th_.emplace_back([this] { loop(); });
-    }
-    ~HostPool() {
-        {
-            std::lock_guard<std::mutex> g(m_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto& t : th_) t.join();
-    }
-    static bool& in_worker() {
-        static thread_local bool flag = false;
-        return flag;
-    }
-    // takes job indices until none is left
-    void work() {
-        for (;;) {
-            size_t t;
-            const std::function<void(size_t)>* job;
-            {
-                std::lock_guard<std::mutex> g(m_);
-                if (!job_ || next_ >= n_) return;
-                t = next_++;
-                job = job_;
-            }
-            (*job)(t);
-            std::lock_guard<std::mutex> g(m_);
-            if (++done_ == n_) done_cv_.notify_all();
-        }
-    }
-    void loop() {
-        in_worker() = true;
-        uint64_t seen = 0;
-        for (;;) {
-            {
-                std::unique_lock<std::mutex> g(m_);
-                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
-            }
-            work();
-        }
-    }
-    std::vector<std::thread> th_;
-    std::mutex m_, busy_;
-    std::condition_variable cv_, done_cv_;
-    const std::function<void(size_t)>* job_ = nullptr;
-    size_t n_ = 0, next_ = 0, done_ = 0;
-    uint64_t gen_ = 0;
-    bool stop_ = false;
-};
-
-// f(begin, end) over [0, n) split into up to 16 chunks of at least
-// kMinPerThread items, run on the host pool.
-template <typename F>
-void parallel_for(size_t n, F&& f, size_t kMinPerThread = 8192) {
-    size_t nt = std::min<size_t>(16, HostPool::get().workers() + 1);
-    nt = std::min(nt, std::max<size_t>(1, n / kMinPerThread));
-    if (nt <= 1) {
-        f((size_t)0, n);
-        return;
-    }
-    const std::function<void(size_t)> job = [&](size_t t) { f(n * t / nt, n * (t + 1) / nt); };
-    if (!HostPool::get().run(nt, job))
-        for (size_t t = 0; t < nt; t++) job(t);
-}
-
-// An allocator whose resize() leaves new elements default-initialised
-// (uninitialised for plain structs), so that a window list of 1.2 GB is not
-// zero-filled before the planner writes every element.
-template <typename T>
-struct NoInitAlloc : std::allocator<T> {
-    template <typename U>
-    struct rebind {
-        using other = NoInitAlloc<U>;
-    };
-    NoInitAlloc() = default;
-    template <typename U>
-    NoInitAlloc(const NoInitAlloc<U>&) {}
-    template <typename U>
-    void construct(U* p) {
-        ::new (static_cast<void*>(p)) U;
-    }
-    template <typename U, typename... Args>
-    void construct(U* p, Args&&... args) {
-        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
-    }
-};
-using DescVec = std::vector<SegDesc, NoInitAlloc<SegDesc>>;
-using LargeVec = std::vector<voss_large_gap, NoInitAlloc<voss_large_gap>>;
-
 // memcpy on several host threads for large blocks (pinned staging of the
 // window descriptors: 1.2 GB for a plain run to 1e14).
 void par_copy(void* dst, const void* src, size_t bytes) {
@@ -313,98 +76,14 @@ void par_copy(void* dst, const void* src, size_t bytes) {
     }, 2);
 }
 
-}  // namespace
-
-struct voss_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    std::string name;
-    int num_cus = 0;
-    double total_gb = 0;
-    voss_tuning tune{131072, 1, 768, 1u << 20, 2, 100, {0, 0}};
-
-    std::vector<uint32_t> primes;
-    std::vector<double> inv_prefix;  // inv_prefix[i] = sum of 1/p over primes[0..i)
-    uint64_t primes_limit = 0;
-    DevBuf<PrimeRec> d_primes;
-    size_t d_nprimes = 0;
-
-    DevBuf<uint32_t> d_pat;
-    uint32_t pat_off[kPresieveGroups] = {};
-    uint32_t pat_len[kPresieveGroups] = {};
-    uint32_t restore0 = 0;
-
-    std::vector<unsigned long long> lg_thresh;
-    DevBuf<unsigned long long> d_lg_thresh;
-
-    ExecBufs eb;                  // buffers of the context's own runs
-    // window list of the last plain run, kept so the next plan reuses its
-    // memory (a fresh 1 MB vector per call costs page faults)
-    DescVec desc_pool;
-    std::vector<hipEvent_t> evs;  // per-round timing events
-    int graph_mode = 1;           // batched runs as one captured hipGraph
-    std::atomic<bool> served{false};  // a running server's scheduler owns it
-
-    // per-window prime counts of the last plain run (gap-sequence sampling)
-    uint64_t last_N = 0, last_seg = 0;
-    uint32_t last_W = 0;
-    std::vector<uint64_t> last_counts;
-    DevBuf<unsigned long long> d_emit;
-    DevBuf<unsigned long long> d_emit_off;
-
-    DevBuf<uint64_t> d_mr_n;
-    DevBuf<uint8_t> d_mr_v;
-    DevBuf<uint32_t> d_surv;
-    DevBuf<uint32_t> d_nsurv;
-    uint32_t* mr_scratch = nullptr;  // the no-allocation MR entry point's lists
-                                     // and counters (device address, fixed)
-    DevBuf<uint64_t> d_vf_pa;   // verify_gaps: p_after, gap, ok
-    DevBuf<int32_t> d_vf_gap;
-    DevBuf<uint8_t> d_vf_ok;
-};
-
-namespace {
-
 // The process-wide gate of library calls (call_gate.h), and whether the
 // calling thread is a server's scheduler.
 CallGate& gate() {
     static CallGate g;
     return g;
 }
+
 thread_local bool t_scheduler = false;
-
-// Admission of one entry point: VOSS_EBUSY when another thread is inside the
-// library, or when the context belongs to a running server (to anyone but
-// that server's scheduler thread).  wait = true blocks for the gate instead
-// (context setup and teardown).
-struct Admit {
-    int rc = VOSS_OK;
-    bool held = false;
-    explicit Admit(const voss_ctx* c, bool wait = false) {
-        if (c && c->served.load() && !t_scheduler) {
-            rc = fail(VOSS_EBUSY, "the context belongs to a running server (submit requests to it)");
-            return;
-        }
-        if (wait) {
-            gate().enter();
-        } else if (!gate().try_enter()) {
-            rc = fail(VOSS_EBUSY,
-                      "another thread is inside the library: one call at a time per process");
-            return;
-        }
-        held = true;
-    }
-    ~Admit() {
-        if (held) gate().leave();
-    }
-    Admit(const Admit&) = delete;
-    Admit& operator=(const Admit&) = delete;
-};
-
-#define VOSS_ADMIT(ctx)          \
-    Admit admit_(ctx);           \
-    if (admit_.rc) return admit_.rc
 
 int set_device(voss_ctx* c) {
     HIPCHK(hipSetDevice(c->device));
@@ -519,28 +198,6 @@ int build_lg_thresh(voss_ctx* c) {
     return VOSS_OK;
 }
 
-// Host-side accumulation of one job (the device accumulators plus the pairs
-// joined on the host: the head of the range and launch boundaries), in the
-// compact bin layout: hist2[h] = gap 2h, gap1 = gap 1 (only (2,3)).
-struct JobHost {
-    int64_t gap1 = 0;
-    std::vector<int64_t> hist2;  // [kHistBins]
-    std::vector<int64_t> mq2;    // [kModqRows * kModqBins]
-    uint64_t count = 0, c1 = 0, mod30[8] = {};
-    uint64_t first = 0, last = 0;
-    uint64_t large_count = 0;
-    bool overflow = false;       // a gap >= 2048 (none exists below 2^64)
-    LargeVec large;
-    // the histograms are allocated on first use (init), so that a batch's
-    // thousands of jobs can be set up on several host threads
-    void init() {
-        if (hist2.empty()) {
-            hist2.assign(kHistBins, 0);
-            mq2.assign(kModqRows * kModqBins, 0);
-        }
-    }
-};
-
 void host_pair(JobHost& J, uint64_t p, uint64_t q, bool full) {
     J.init();
     const uint64_t g = q - p;
@@ -643,7 +300,7 @@ void sort_large(LargeVec& v) {
 }
 
 // The canonical order of sort_large as a comparison.
-inline bool large_before(const voss_large_gap& x, const voss_large_gap& y) {
+bool large_before(const voss_large_gap& x, const voss_large_gap& y) {
     return x.merit > y.merit || (x.merit == y.merit && x.position < y.position);
 }
 
@@ -693,11 +350,6 @@ void wheel_modq(const JobHost& J, uint64_t* modq) {
         modq[6 + kW30[o]] += J.mod30[o];
     }
 }
-
-struct Plan {
-    DescVec desc;
-    uint32_t njobs = 0;
-};
 
 // Windows of the grid w0 = k*W covering numbers [a, b] (a >= 7): k from
 // ka = (a/30 - origin)/W to kb = (b/30 - origin)/W, each window non-empty
@@ -785,20 +437,6 @@ SieveArgs make_args(voss_ctx* c, ExecBufs& B, uint32_t ndesc, unsigned long long
     return a;
 }
 
-constexpr size_t kMaxDescPerLaunch = 1u << 20;
-
-struct ExecTimes {
-    double device_ms = 0;  // the whole device sequence
-    double sieve_ms = 0;   // sieve kernels (-1: not timed, under a graph)
-    double bucket_ms = 0;  // bucket kernels (-1: not timed, under a graph)
-    uint64_t bucket_entries = 0;
-    uint32_t rounds = 0;
-    uint32_t reruns = 0;   // re-executions after a bucket or large-list overflow
-};
-
-// Bucket entries per round (8 GiB of HBM); offsets stay 32-bit.
-constexpr uint64_t kBucketEntryBudget = 1ull << 31;
-
 // Room for the entries of one bucket (a window of W bytes) fed by primes
 // with sum(1/p) = s.  A prime p has on average 8W/p multiples coprime to 30
 // among the window's 30W numbers; the window's count is a sum of nearly
@@ -810,19 +448,6 @@ uint32_t bucket_room(double s, uint32_t W, uint32_t pct) {
     const double r = (e + 8.0 * std::sqrt(e) + 64.0) * pct / 100.0;
     return (uint32_t)(((uint64_t)std::ceil(r) + 15) / 16 * 16);
 }
-
-// A round is a run of consecutive windows of the plan launched together:
-// bucket fill, sieve, stitch.  Windows of jobs with at least
-// bucket_min_windows windows get a bucket for the primes >= bucket_limit.
-struct Round {
-    size_t off = 0, nd = 0;        // plan windows [off, off + nd)
-    DescVec desc;                  // with bucket / nprimes set for this round
-    std::vector<BucketTask> tasks;
-    uint32_t nbuckets = 0;
-    std::vector<uint32_t> room;    // entries per bucket
-    size_t task_off = 0, bkt_off = 0, boff_off = 0;  // places in the staged arrays
-    uint64_t room_total = 0;
-};
 
 // Estimated VALU work of a bucket task: per-prime setup (a class-mode
 // prime also starts eight classes), class-mode hits, and walk steps (30
@@ -983,481 +608,9 @@ int log2_window(const voss_ctx* c) {
     return l;
 }
 
-// Runs a plan on the device: every round's bucket fill, sieve and stitch
-// in stream order, then the accumulators, window records and flags back
-// to pinned memory, with a single synchronisation at the end.  The direct
-// sequence has no host step inside, so it can be captured in a hipGraph
-// and replayed.  Each job's accumulators sum over all its rounds on the
-// device; the pairs across round edges are joined on the host from the
-// window records.
-struct Executor {
-    voss_ctx* c = nullptr;
-    ExecBufs* B = nullptr;
-    Plan P;
-    std::vector<Round> rounds;
-    size_t ntask = 0, nbkt = 0, nboff = 0;
-    uint64_t lcap = 0;         // device large-gap list capacity
-    bool staged = false;       // buffers sized and staging arrays filled
-    bool exact = false;        // bucket rooms are the exact counts
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    const PrimeRec* graph_primes = nullptr;  // base-prime table the graph reads
-    bool copy_acc = true;      // read the accumulators back after a run
-    uint64_t nlarge = 0;       // large-gap records of the last run (in B->h_large)
-    // Miller-Rabin batches run beside the rounds on a second stream (forked
-    // from and joined back into the main one, so a capture holds both)
-    struct MrJob {
-        const uint64_t* d_n;
-        uint64_t count;
-        uint8_t* d_verdict;
-        uint32_t* d_surv;
-        uint32_t* d_nsurv;
-    };
-    std::vector<MrJob> mr;
-    hipStream_t side = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-
-    Executor(voss_ctx* ctx, ExecBufs* bufs) : c(ctx), B(bufs) {}
-    ~Executor() {
-        drop_graph();
-        if (fork_ev) (void)hipEventDestroy(fork_ev);
-        if (join_ev) (void)hipEventDestroy(join_ev);
-        if (side) (void)hipStreamDestroy(side);
-    }
-
-    int ensure_side() {
-        if (side) return VOSS_OK;
-        HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
-        return VOSS_OK;
-    }
-
-    // fork: the side stream waits for the main one, runs the MR batches
-    int enqueue_mr_fork() {
-        if (mr.empty()) return VOSS_OK;
-        HIPCHK(hipEventRecord(fork_ev, c->stream));
-        HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
-        for (const MrJob& m : mr)
-            HIPCHK(launch_mr(m.d_n, m.count, m.d_verdict, m.d_surv, m.d_nsurv, c->num_cus, side));
-        return VOSS_OK;
-    }
-
-    // join: the main stream waits for the side stream
-    int enqueue_mr_join() {
-        if (mr.empty()) return VOSS_OK;
-        HIPCHK(hipEventRecord(join_ev, side));
-        HIPCHK(hipStreamWaitEvent(c->stream, join_ev, 0));
-        return VOSS_OK;
-    }
-    Executor(const Executor&) = delete;
-    Executor& operator=(const Executor&) = delete;
-
-    void drop_graph() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        gexec = nullptr;
-        graph = nullptr;
-    }
-
-    void plan_rounds() {
-        rounds.clear();
-        make_rounds(c, P, rounds);
-        ntask = nbkt = nboff = 0;
-        for (Round& R : rounds) {
-            R.task_off = ntask;
-            R.bkt_off = nbkt;
-            R.boff_off = nboff;
-            ntask += R.tasks.size();
-            nbkt += R.nbuckets;
-            nboff += R.nbuckets + 1;
-        }
-        staged = false;
-        exact = false;
-        drop_graph();
-    }
-
-    // Sizes every buffer and fills the pinned staging arrays (descriptors,
-    // tasks, bucket offsets from the rooms).  Nothing is allocated after
-    // this until the next staging, so captured pointers stay valid.
-    int stage() {
-        const size_t nd = P.desc.size();
-        const uint32_t nj = std::max<uint32_t>(P.njobs, 1);
-        if (lcap == 0) {
-            // room for every large-gap candidate: about 1.5e-5 of the primes
-            // at 1e11..1e14 (8.9e-6 at 1e11, 1.3e-5 at 1e13); 2e-5 of a
-            // prime-count estimate avoids a re-run for overflow in practice
-            // (a sample of at most ~65536 windows, scaled)
-            const size_t step = std::max<size_t>(16, P.desc.size() >> 16);
-            double primes = 0;
-            for (size_t i = 0; i < P.desc.size(); i += step) {
-                const SegDesc& d = P.desc[i];
-                primes += (double)step * (double)(d.n_hi - d.n_lo + 1) /
-                          std::log((double)d.n_hi + 3.0);
-            }
-            lcap = std::max<uint64_t>({B->d_large.cap, 1u << 20, (uint64_t)(2e-5 * primes)});
-        }
-        uint64_t ent = 1;
-        uint32_t nb_max = 1;
-        for (const Round& R : rounds) {
-            ent = std::max<uint64_t>(ent, R.room_total);
-            nb_max = std::max<uint32_t>(nb_max, R.nbuckets);
-        }
-        if (ent >= 0xFFFFFFFFull) return fail(VOSS_ENOMEM, "bucket round beyond 2^32 entries");
-        drop_graph();
-        HIPCHK(B->d_desc.reserve(nd));
-        HIPCHK(B->d_rec.reserve(nd));
-        HIPCHK(B->d_acc.reserve(nj));
-        HIPCHK(B->d_large.reserve(lcap));
-        HIPCHK(B->d_flags.reserve(4));
-        HIPCHK(B->d_btask.reserve(ntask));
-        HIPCHK(B->d_bcnt.reserve(nb_max));
-        HIPCHK(B->d_bcur.reserve(nbkt));
-        HIPCHK(B->d_boff.reserve(nboff));
-        HIPCHK(B->d_bent.reserve(ent));
-        HIPCHK(B->h_desc.reserve(nd));
-        HIPCHK(B->h_btask.reserve(ntask));
-        HIPCHK(B->h_boff.reserve(nboff));
-        HIPCHK(B->h_bcnt.reserve(nb_max));
-        HIPCHK(B->h_rec.reserve(nd));
-        HIPCHK(B->h_acc.reserve(nj));
-        HIPCHK(B->h_flags.reserve(2));
-        for (const Round& R : rounds) {
-            par_copy(B->h_desc.p + R.off, R.desc.empty() ? P.desc.data() + R.off : R.desc.data(),
-                     R.nd * sizeof(SegDesc));
-            if (!R.tasks.empty())
-                std::memcpy(B->h_btask.p + R.task_off, R.tasks.data(),
-                            R.tasks.size() * sizeof(BucketTask));
-            unsigned long long o = 0;
-            for (uint32_t b = 0; b < R.nbuckets; b++) {
-                B->h_boff.p[R.boff_off + b] = o;
-                o += R.room[b];
-            }
-            B->h_boff.p[R.boff_off + R.nbuckets] = o;
-        }
-        staged = true;
-        return VOSS_OK;
-    }
-
-    hipEvent_t ev(size_t i) { return c->evs[i]; }
-
-    uint32_t* task_counter() { return reinterpret_cast<uint32_t*>(B->d_flags.p + 2); }
-    uint32_t* win_counter() { return reinterpret_cast<uint32_t*>(B->d_flags.p + 3); }
-
-    int ensure_events(size_t n) {
-        while (c->evs.size() < n) {
-            hipEvent_t e;
-            HIPCHK(hipEventCreate(&e));
-            c->evs.push_back(e);
-        }
-        return VOSS_OK;
-    }
-
-    SieveArgs round_args(const Round& R) {
-        SieveArgs a = make_args(c, *B, (uint32_t)R.nd, lcap);
-        a.desc = B->d_desc.p + R.off;
-        a.rec = B->d_rec.p + R.off;
-        a.bucket_off = B->d_boff.p + R.boff_off;
-        a.bucket_cnt = B->d_bcur.p + R.bkt_off;
-        return a;
-    }
-
-    // The direct sequence (no host step): capturable.  timed records four
-    // events per round (not under capture).
-    int enqueue_direct(bool timed) {
-        hipStream_t s = c->stream;
-        const size_t nd = P.desc.size();
-        const int log2w = log2_window(c);
-        const uint32_t grid = (uint32_t)c->num_cus * std::max<uint32_t>(c->tune.wg_per_cu, 1);
-        const uint32_t bgrid = (uint32_t)c->num_cus * 2;
-        if (nd)
-            HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, nd * sizeof(SegDesc),
-                                  hipMemcpyHostToDevice, s));
-        if (ntask) {
-            HIPCHK(hipMemcpyAsync(B->d_btask.p, B->h_btask.p, ntask * sizeof(BucketTask),
-                                  hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(B->d_boff.p, B->h_boff.p, nboff * 8, hipMemcpyHostToDevice, s));
-            HIPCHK(launch_fill_u32(B->d_bcur.p, nbkt, 0, s));
-        }
-        HIPCHK(launch_fill_u32((uint32_t*)B->d_flags.p, 4, 0, s));
-        HIPCHK(launch_init_acc(B->d_acc.p, P.njobs, s));
-        int rc = enqueue_mr_fork();
-        if (rc) return rc;
-        for (size_t r = 0; r < rounds.size(); r++) {
-            const Round& R = rounds[r];
-            if (timed) HIPCHK(hipEventRecord(ev(4 * r), s));
-            if (!R.tasks.empty()) {
-                HIPCHK(launch_fill_u32(task_counter(), 1, 0, s));
-                HIPCHK(launch_bucket_fill(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
-                                          c->d_primes.p, B->d_boff.p + R.boff_off,
-                                          B->d_bcur.p + R.bkt_off, B->d_bent.p,
-                                          (uint32_t*)(B->d_flags.p + 1), task_counter(), log2w,
-                                          bgrid, s));
-            }
-            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 1), s));
-            const SieveArgs a = round_args(R);
-            HIPCHK(launch_fill_u32(win_counter(), 1, 0, s));
-            HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, s));
-            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 2), s));
-            HIPCHK(launch_stitch(a, s));
-            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 3), s));
-        }
-        if ((rc = enqueue_mr_join())) return rc;
-        return enqueue_readback();
-    }
-
-    int enqueue_readback() {
-        hipStream_t s = c->stream;
-        if (copy_acc && P.njobs)
-            HIPCHK(hipMemcpyAsync(B->h_acc.p, B->d_acc.p, P.njobs * sizeof(JobAcc),
-                                  hipMemcpyDeviceToHost, s));
-        if (!P.desc.empty())
-            HIPCHK(hipMemcpyAsync(B->h_rec.p, B->d_rec.p, P.desc.size() * sizeof(SegRec),
-                                  hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(B->h_flags.p, B->d_flags.p, 16, hipMemcpyDeviceToHost, s));
-        return VOSS_OK;
-    }
-
-    // Exact sequence: per round, a count pass, the host prefix sum, then the
-    // fill with exact offsets.  The rooms become the exact counts, so later
-    // direct runs of this plan cannot overflow.
-    int enqueue_exact(bool timed) {
-        hipStream_t s = c->stream;
-        const size_t nd = P.desc.size();
-        const int log2w = log2_window(c);
-        const uint32_t grid = (uint32_t)c->num_cus * std::max<uint32_t>(c->tune.wg_per_cu, 1);
-        const uint32_t bgrid = (uint32_t)c->num_cus * 2;
-        HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, nd * sizeof(SegDesc),
-                              hipMemcpyHostToDevice, s));
-        if (ntask)
-            HIPCHK(hipMemcpyAsync(B->d_btask.p, B->h_btask.p, ntask * sizeof(BucketTask),
-                                  hipMemcpyHostToDevice, s));
-        HIPCHK(launch_fill_u32((uint32_t*)B->d_flags.p, 4, 0, s));
-        HIPCHK(launch_init_acc(B->d_acc.p, P.njobs, s));
-        for (size_t r = 0; r < rounds.size(); r++) {
-            Round& R = rounds[r];
-            if (timed) HIPCHK(hipEventRecord(ev(4 * r), s));
-            if (!R.tasks.empty()) {
-                HIPCHK(launch_fill_u32(B->d_bcnt.p, R.nbuckets, 0, s));
-                HIPCHK(launch_fill_u32(task_counter(), 1, 0, s));
-                HIPCHK(launch_bucket_count(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
-                                           c->d_primes.p, B->d_bcnt.p, task_counter(), log2w,
-                                           bgrid, s));
-                HIPCHK(hipMemcpyAsync(B->h_bcnt.p, B->d_bcnt.p, R.nbuckets * 4,
-                                      hipMemcpyDeviceToHost, s));
-                HIPCHK(hipStreamSynchronize(s));
-                unsigned long long o = 0;
-                for (uint32_t b = 0; b < R.nbuckets; b++) {
-                    B->h_boff.p[R.boff_off + b] = o;
-                    R.room[b] = (B->h_bcnt.p[b] + 3u) & ~3u;  // 16-byte aligned buckets
-                    o += R.room[b];
-                }
-                B->h_boff.p[R.boff_off + R.nbuckets] = o;
-                R.room_total = o;
-                if (o >= 0xFFFFFFFFull) return fail(VOSS_ENOMEM, "bucket round beyond 2^32 entries");
-                // the stream is idle here: growing the entry buffer is safe
-                HIPCHK(B->d_bent.reserve(std::max<uint64_t>(o, 1)));
-                HIPCHK(hipMemcpyAsync(B->d_boff.p + R.boff_off, B->h_boff.p + R.boff_off,
-                                      (R.nbuckets + 1) * 8, hipMemcpyHostToDevice, s));
-                HIPCHK(launch_fill_u32(B->d_bcur.p + R.bkt_off, R.nbuckets, 0, s));
-                HIPCHK(launch_fill_u32(task_counter(), 1, 0, s));
-                HIPCHK(launch_bucket_fill(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
-                                          c->d_primes.p, B->d_boff.p + R.boff_off,
-                                          B->d_bcur.p + R.bkt_off, B->d_bent.p,
-                                          (uint32_t*)(B->d_flags.p + 1), task_counter(), log2w,
-                                          bgrid, s));
-            }
-            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 1), s));
-            const SieveArgs a = round_args(R);
-            HIPCHK(launch_fill_u32(win_counter(), 1, 0, s));
-            HIPCHK(launch_sieve(a, (int)c->tune.window_bytes, grid, s));
-            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 2), s));
-            HIPCHK(launch_stitch(a, s));
-            if (timed) HIPCHK(hipEventRecord(ev(4 * r + 3), s));
-        }
-        return enqueue_readback();
-    }
-
-    int capture() {
-        drop_graph();
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_direct(false);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        HIPCHK(e);
-        graph = g;
-        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-        graph_primes = c->d_primes.p;
-        return VOSS_OK;
-    }
-
-    // Executes the plan until it completes without overflow.  On return
-    // B->h_acc, B->h_rec and B->h_large (nlarge records) hold the results.
-    int run(bool use_graph, ExecTimes* tm) {
-        ExecTimes t;
-        t.rounds = (uint32_t)rounds.size();
-        int rc;
-        if ((rc = ensure_events(4 * rounds.size()))) return rc;
-        if (gexec && graph_primes != c->d_primes.p) drop_graph();  // base primes moved
-        bool breakdown = false;
-        for (;;) {
-            if (!staged && (rc = stage())) return rc;
-            const bool timed = !use_graph;
-            HIPCHK(hipEventRecord(c->ev0, c->stream));
-            if (use_graph) {
-                if (!gexec && (rc = capture())) return rc;
-                HIPCHK(hipGraphLaunch(gexec, c->stream));
-            } else if ((rc = enqueue_direct(timed))) {
-                return rc;
-            }
-            HIPCHK(hipEventRecord(c->ev1, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-            breakdown = timed;
-            if (B->h_flags.p[1]) {
-                // a bucket outgrew its room: exact counts, then direct again
-                t.reruns++;
-                HIPCHK(hipEventRecord(c->ev0, c->stream));
-                breakdown = true;
-                if ((rc = enqueue_exact(true))) return rc;
-                HIPCHK(hipEventRecord(c->ev1, c->stream));
-                HIPCHK(hipStreamSynchronize(c->stream));
-                staged = false;  // re-stage with the exact rooms next time
-                exact = true;
-                if (B->h_flags.p[1]) return fail(VOSS_EDEVICE, "bucket overflow with exact counts");
-            }
-            if (B->h_flags.p[0] > lcap) {
-                // keep every candidate (canonical top-K needs all of them)
-                t.reruns++;
-                lcap = B->h_flags.p[0] + (B->h_flags.p[0] >> 2);
-                staged = false;
-                continue;
-            }
-            break;
-        }
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        t.device_ms = ms;
-        if (!breakdown) {
-            t.sieve_ms = t.bucket_ms = -1;
-        } else {
-            for (size_t r = 0; r < rounds.size(); r++) {
-                float b = 0, sv = 0;
-                HIPCHK(hipEventElapsedTime(&b, ev(4 * r), ev(4 * r + 1)));
-                HIPCHK(hipEventElapsedTime(&sv, ev(4 * r + 1), ev(4 * r + 2)));
-                t.bucket_ms += b;
-                t.sieve_ms += sv;
-            }
-        }
-        for (const Round& R : rounds) t.bucket_entries += R.room_total;
-        nlarge = B->h_flags.p[0];
-        HIPCHK(B->h_large.reserve(std::max<uint64_t>(nlarge, 1)));
-        if (nlarge) {
-            HIPCHK(hipMemcpyAsync(B->h_large.p, B->d_large.p, nlarge * sizeof(LargeRaw),
-                                  hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-        }
-        if (tm) *tm = t;
-        return VOSS_OK;
-    }
-
-    // Folds the results into per-job host accumulators: device sums, large
-    // records, and the pairs across round edges.
-    int collect(std::vector<JobHost>& jobs, std::vector<uint64_t>* counts) {
-        jobs.assign(P.njobs, JobHost());
-        for (uint32_t j = 0; j < P.njobs; j++)
-            if (B->h_acc.p[j].error) return fail(VOSS_ERANGE, "prime gap >= 2048 (beyond the device histogram)");
-        parallel_for(P.njobs, [&](size_t j0, size_t j1) {
-            for (size_t j = j0; j < j1; j++) add_acc(jobs[j], B->h_acc.p[j]);
-        }, 256);
-        // large records with their merits (one log each, on several host
-        // threads for long lists), straight into the job lists: a single
-        // job's list in device order, several jobs' via per-job offsets
-        const LargeRaw* raw = B->h_large.p;
-        auto fill = [&](voss_large_gap* dst, const uint32_t* at) {
-            parallel_for(nlarge, [&](size_t k0, size_t k1) {
-                for (size_t k = k0; k < k1; k++) {
-                    const LargeRaw& r = raw[k];
-                    dst[at ? at[k] : k] = {r.p_before + r.gap, (int32_t)r.gap,
-                                           merit_of(r.p_before, r.gap)};
-                }
-            });
-        };
-        if (P.njobs == 1) {
-            jobs[0].large.resize(nlarge);
-            fill(jobs[0].large.data(), nullptr);
-        } else if (nlarge) {
-            std::vector<uint32_t> n_of(P.njobs, 0), at(nlarge);
-            for (uint64_t k = 0; k < nlarge; k++) at[k] = n_of[raw[k].job]++;
-            LargeVec all(nlarge);
-            std::vector<uint64_t> first(P.njobs + 1, 0);
-            for (uint32_t j = 0; j < P.njobs; j++) first[j + 1] = first[j] + n_of[j];
-            for (uint64_t k = 0; k < nlarge; k++) at[k] += (uint32_t)first[raw[k].job];
-            fill(all.data(), at.data());
-            for (uint32_t j = 0; j < P.njobs; j++)
-                if (n_of[j])
-                    jobs[j].large.assign(all.begin() + first[j], all.begin() + first[j + 1]);
-        }
-        const SegRec* rec = B->h_rec.p;
-        if (counts) {
-            counts->resize(P.desc.size());
-            parallel_for(P.desc.size(), [&](size_t i0, size_t i1) {
-                for (size_t i = i0; i < i1; i++) (*counts)[i] = rec[i].count;
-            }, 1u << 18);
-        }
-        // round edge: pair the last prime of a job before the round with the
-        // first non-empty window of the same job in the round (nothing to do
-        // for a single round: the stitch kernel joined all its windows)
-        if (rounds.size() < 2) return VOSS_OK;
-        if (P.njobs == 1) {
-            // one job: only the windows next to each edge matter, found by
-            // scanning from the edge (plain runs to 1e14 have 2.5e7 windows)
-            bool has = false;
-            uint64_t last = 0;
-            uint32_t lseg = 0;
-            for (const Round& R : rounds) {
-                size_t i = R.off;
-                const size_t e = R.off + R.nd;
-                while (i < e && rec[i].count == 0) i++;
-                if (i == e) continue;
-                if (has) host_pair(jobs[0], last, rec[i].first, lseg == P.desc[i].lseg);
-                size_t k = e - 1;
-                while (rec[k].count == 0) k--;  // stops at i at the latest
-                has = true;
-                last = rec[k].last;
-                lseg = P.desc[k].lseg;
-            }
-            return VOSS_OK;
-        }
-        std::vector<uint64_t> carry_last(P.njobs, 0);
-        std::vector<uint32_t> carry_lseg(P.njobs, 0);
-        std::vector<uint8_t> carry_has(P.njobs, 0), seen(P.njobs, 0);
-        for (const Round& R : rounds) {
-            std::fill(seen.begin(), seen.end(), 0);
-            for (size_t i = R.off; i < R.off + R.nd; i++) {
-                const SegDesc& d = P.desc[i];
-                if (rec[i].count == 0) continue;
-                if (!seen[d.job]) {
-                    seen[d.job] = 1;
-                    if (carry_has[d.job])
-                        host_pair(jobs[d.job], carry_last[d.job], rec[i].first,
-                                  carry_lseg[d.job] == d.lseg);
-                }
-                carry_has[d.job] = 1;
-                carry_last[d.job] = rec[i].last;
-                carry_lseg[d.job] = d.lseg;
-            }
-        }
-        return VOSS_OK;
-    }
-};
-
 // One-shot execution with the context's buffers.
 int execute(voss_ctx* c, Plan&& P, std::vector<JobHost>& jobs, ExecTimes* times,
-            std::vector<uint64_t>* counts = nullptr, bool use_graph = false) {
+            std::vector<uint64_t>* counts, bool use_graph) {
     jobs.assign(P.njobs, JobHost());
     if (P.njobs == 0 || P.desc.empty()) {
         if (times) *times = ExecTimes();
@@ -1485,7 +638,7 @@ int check_result(voss_result* out) {
 
 // Zeroes every field but the caller's large-gap buffer.  The two big
 // histograms are left to write_result, which fills them completely.
-void clear_result(voss_result* out, bool histograms = true) {
+void clear_result(voss_result* out, bool histograms) {
     voss_large_gap* lg = out->large;
     const uint64_t cap = out->large_capacity;
     if (histograms) {
@@ -1498,13 +651,6 @@ void clear_result(voss_result* out, bool histograms = true) {
     out->large = lg;
     out->large_capacity = cap;
 }
-
-// Interval bookkeeping shared by the single and batched forms: primes
-// 2, 3, 5 of [lo, hi) precede the wheel primes.
-struct IntervalHead {
-    uint64_t total = 0, first = 0, last = 0, c1 = 0, c3 = 0;
-    uint64_t modq[VOSS_Q_SLOTS] = {};
-};
 
 IntervalHead finish_interval_common(JobHost& J, uint64_t lo, uint64_t hi) {
     IntervalHead H;
@@ -1571,7 +717,8 @@ void finish_interval(JobHost& J, uint64_t lo, uint64_t hi, voss_interval_result*
     }
 }
 
-}  // namespace
+}  // namespace core
+}  // namespace voss
 
 extern "C" {
 
@@ -1636,6 +783,7 @@ void voss_destroy(voss_ctx* c) {
     c->d_vf_pa.release();
     c->d_vf_gap.release();
     c->d_vf_ok.release();
+    if (c->serve_cache) c->free_serve_cache(c->serve_cache);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -1719,7 +867,7 @@ int voss_autotune(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
     const std::vector<uint32_t> tiers = {512, 768, 1024, 2048};
     const std::vector<uint32_t> blimits = {1u << 17, 1u << 18, 1u << 19, 1u << 20, 1u << 21,
                                            1u << 22, 1u << 23, 1u << 25, 0xFFFFFFFFu};
-    const std::vector<uint32_t> bmins = {2, 4, 8};
+    const std::vector<uint32_t> bmins = {1, 2, 4, 8};
     for (int pass = 0; pass < 2; pass++) {
         bool moved = false;
         for (int dim = 0; dim < 4; dim++) {
@@ -2448,418 +1596,6 @@ int voss_memcpy_d2h(voss_ctx* c, void* dst, const void* src, size_t bytes) {
     if (rc) return rc;
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return VOSS_OK;
-}
-
-}  // extern "C"
-
-// ------------------------------------------------------------------ serving
-
-#include <condition_variable>
-#include <deque>
-#include <memory>
-#include <thread>
-#include <unordered_map>
-
-namespace {
-
-struct ServeReq {
-    uint64_t id = 0;
-    bool is_mr = false;
-    uint64_t lo = 0, hi = 0;
-    std::vector<uint64_t> cand;
-    std::vector<uint8_t> verdict;
-    double t_submit = 0, t_done = 0;
-    uint64_t hbm = 0;
-    // interval requests
-    DescVec win;  // planned windows, job = slot
-    size_t next = 0;
-    int slot = -1;
-    JobHost J;
-    uint64_t carry_last = 0;
-    bool carry_has = false;
-    voss_interval_result res{};
-    std::vector<voss_large_gap> large;
-    int status = VOSS_OK;
-    bool done = false;
-};
-
-using ReqPtr = std::shared_ptr<ServeReq>;
-
-}  // namespace
-
-struct voss_server {
-    voss_ctx* c = nullptr;
-    voss_serve_config cfg{};
-    std::thread th;
-    std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<ReqPtr> queue;
-    std::unordered_map<uint64_t, ReqPtr> all;
-    uint64_t next_id = 1;
-    bool stopping = false;
-    int err = VOSS_OK;
-    std::string errmsg;
-    // owned by the scheduler thread
-    std::vector<ReqPtr> active;
-    std::vector<int> free_slots;
-    uint64_t hbm_used = 0;
-    DevBuf<JobAcc> d_acc;
-    HostBuf<JobAcc> h_acc;
-    hipStream_t mr_stream = nullptr;
-    DevBuf<uint64_t> d_mr_n;
-    DevBuf<uint8_t> d_mr_v;
-    DevBuf<uint32_t> d_surv;
-    DevBuf<uint32_t> d_nsurv;
-};
-
-namespace {
-
-constexpr uint64_t kSlotBytes = sizeof(JobAcc);
-
-uint64_t req_hbm(const voss_server* s, const ServeReq& r) {
-    if (r.is_mr) return r.cand.size() * (8 + 1 + 4);
-    return kSlotBytes + (uint64_t)s->cfg.large_capacity * sizeof(LargeRaw);
-}
-
-// One sieve step over windows of several resident requests: buckets, sieve
-// and stitch per round, accumulators left in the requests' HBM slots; the
-// pairs across round/step edges and the large-gap candidates go to the
-// requests on the host.
-int serve_sieve_step(voss_server* s, const std::vector<std::pair<ReqPtr, size_t>>& take) {
-    voss_ctx* c = s->c;
-    Executor X(c, &c->eb);
-    X.P.njobs = s->cfg.max_active;
-    X.copy_acc = false;
-    std::vector<ServeReq*> by_slot(s->cfg.max_active, nullptr);
-    for (const auto& t : take) {
-        ServeReq* r = t.first.get();
-        by_slot[r->slot] = r;
-        for (size_t k = 0; k < t.second; k++) X.P.desc.push_back(r->win[r->next + k]);
-    }
-    if (X.P.desc.empty()) return VOSS_OK;
-    X.plan_rounds();
-    int rc = X.run(false, nullptr);
-    if (rc) return rc;
-    // the step's sums into the requests' resident slots (stream order puts
-    // this before any later read of a slot)
-    HIPCHK(launch_add_acc(s->d_acc.p, c->eb.d_acc.p, s->cfg.max_active, c->stream));
-    for (uint64_t k = 0; k < X.nlarge; k++) {
-        const LargeRaw& x = c->eb.h_large.p[k];
-        ServeReq* r = by_slot[x.job];
-        r->J.large.push_back({x.p_before + x.gap, (int32_t)x.gap, merit_of(x.p_before, x.gap)});
-    }
-    // edges between each round and what the request saw before it
-    const SegRec* rec = c->eb.h_rec.p;
-    std::vector<uint8_t> seen(s->cfg.max_active, 0);
-    for (const Round& R : X.rounds) {
-        std::fill(seen.begin(), seen.end(), 0);
-        for (size_t i = R.off; i < R.off + R.nd; i++) {
-            const SegDesc& d = X.P.desc[i];
-            if (rec[i].count == 0) continue;
-            ServeReq* r = by_slot[d.job];
-            if (!seen[d.job]) {
-                seen[d.job] = 1;
-                if (r->carry_has) host_pair(r->J, r->carry_last, rec[i].first, true);
-            }
-            r->carry_has = true;
-            r->carry_last = rec[i].last;
-        }
-    }
-    return VOSS_OK;
-}
-
-void serve_fail_all(voss_server* s, int code, const std::string& msg) {
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->err = code;
-    s->errmsg = msg;
-    for (auto& kv : s->all)
-        if (!kv.second->done) {
-            kv.second->status = code;
-            kv.second->done = true;
-        }
-    s->cv_done.notify_all();
-}
-
-void serve_loop(voss_server* s) {
-    voss_ctx* c = s->c;
-    t_scheduler = true;
-    if (hipSetDevice(c->device) != hipSuccess) {
-        serve_fail_all(s, VOSS_EDEVICE, "hipSetDevice");
-        return;
-    }
-    const uint32_t W = c->tune.window_bytes;
-    while (true) {
-        std::vector<ReqPtr> admitted_iv, admitted_mr;
-        {
-            std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stopping || !s->queue.empty() || !s->active.empty(); });
-            if (s->queue.empty() && s->active.empty() && s->stopping) break;
-            // admission by slots and HBM budget, in arrival order
-            uint64_t mr_take = 0;
-            while (!s->queue.empty()) {
-                ReqPtr r = s->queue.front();
-                const uint64_t cost = req_hbm(s, *r);
-                const bool fits = s->hbm_used + cost <= s->cfg.hbm_budget_bytes ||
-                                  (s->active.empty() && admitted_iv.empty() && admitted_mr.empty());
-                if (!fits) break;
-                if (r->is_mr) {
-                    if (!admitted_mr.empty() && mr_take + r->cand.size() > s->cfg.mr_per_step) break;
-                    mr_take += r->cand.size();
-                    admitted_mr.push_back(r);
-                } else {
-                    if (s->free_slots.empty()) break;
-                    r->slot = s->free_slots.back();
-                    s->free_slots.pop_back();
-                    admitted_iv.push_back(r);
-                    s->active.push_back(r);
-                }
-                r->hbm = cost;
-                s->hbm_used += cost;
-                s->queue.pop_front();
-            }
-        }
-        // the step's device work holds the process-wide call gate, so no
-        // other thread's call runs beside it (it waits for one to finish)
-        struct StepGate {
-            StepGate() { gate().enter(); }
-            ~StepGate() { gate().leave(); }
-        } step_gate;
-        int rc = VOSS_OK;
-        // plan the admitted interval requests; fresh accumulator slots
-        for (auto& r : admitted_iv) {
-            if (r->hi > 7) {
-                if ((rc = ensure_primes(c, r->hi / 30 * 30 + 30ull * W + 30))) break;
-                Plan P1;
-                P1.njobs = 1;
-                const uint64_t a = std::max<uint64_t>(r->lo, 7), b = r->hi - 1;
-                plan_range(c, P1, a, b, (uint32_t)r->slot, 0, W, (a / 30) & ~4095ull);
-                r->win.swap(P1.desc);
-            }
-            if (hipSuccess != launch_init_acc(s->d_acc.p + r->slot, 1, c->stream)) {
-                rc = fail(VOSS_EDEVICE, "init_acc");
-                break;
-            }
-        }
-        // Miller-Rabin batch on the second stream (overlaps the sieve step)
-        uint64_t mr_n = 0;
-        std::vector<uint64_t> all_n;  // source of the async copy: lives past the step
-        if (rc == VOSS_OK && !admitted_mr.empty()) {
-            for (auto& r : admitted_mr) mr_n += r->cand.size();
-            all_n.reserve(mr_n);
-            for (auto& r : admitted_mr) all_n.insert(all_n.end(), r->cand.begin(), r->cand.end());
-            if (s->d_mr_n.reserve(mr_n) || s->d_mr_v.reserve(mr_n) || s->d_surv.reserve(2 * mr_n) ||
-                s->d_nsurv.reserve(3))
-                rc = fail(VOSS_ENOMEM, "MR buffers");
-            else if (hipMemcpyAsync(s->d_mr_n.p, all_n.data(), mr_n * 8, hipMemcpyHostToDevice,
-                                    s->mr_stream) != hipSuccess ||
-                     launch_mr(s->d_mr_n.p, mr_n, s->d_mr_v.p, s->d_surv.p, s->d_nsurv.p,
-                               c->num_cus, s->mr_stream) != hipSuccess)
-                rc = fail(VOSS_EDEVICE, "MR launch");
-        }
-        // sieve step: windows round-robin over the resident requests
-        std::vector<std::pair<ReqPtr, size_t>> take;
-        if (rc == VOSS_OK && !s->active.empty()) {
-            size_t budget = s->cfg.windows_per_step;
-            const size_t n_act = s->active.size();
-            const size_t quota = std::max<size_t>(1, budget / n_act);
-            for (auto& r : s->active) {
-                const size_t left = r->win.size() - r->next;
-                const size_t k = std::min(left, quota);
-                take.push_back({r, k});
-            }
-            rc = serve_sieve_step(s, take);
-            for (auto& t : take) t.first->next += t.second;
-        }
-        if (mr_n && hipStreamSynchronize(s->mr_stream) != hipSuccess && rc == VOSS_OK)
-            rc = fail(VOSS_EDEVICE, "MR sync");
-        if (rc != VOSS_OK) {
-            serve_fail_all(s, rc, voss_last_error());
-            return;
-        }
-        // completions
-        std::vector<ReqPtr> finished;
-        for (size_t i = 0; i < s->active.size();) {
-            ReqPtr r = s->active[i];
-            if (r->next < r->win.size()) { i++; continue; }
-            if (s->h_acc.reserve(1) ||
-                hipMemcpy(s->h_acc.p, s->d_acc.p + r->slot, sizeof(JobAcc), hipMemcpyDeviceToHost)) {
-                serve_fail_all(s, VOSS_EDEVICE, "reading a request's accumulators");
-                return;
-            }
-            if (!r->win.empty()) add_acc(r->J, *s->h_acc.p);
-            r->large.resize(s->cfg.large_capacity);
-            r->res.large = r->large.data();
-            r->res.large_capacity = s->cfg.large_capacity;
-            finish_interval(r->J, r->lo, r->hi, &r->res);
-            r->large.resize(std::min<uint64_t>(r->res.large_count, s->cfg.large_capacity));
-            if (r->J.overflow) r->status = VOSS_ERANGE;
-            finished.push_back(r);
-            s->active.erase(s->active.begin() + (long)i);
-        }
-        for (auto& r : admitted_mr) {
-            r->verdict.resize(r->cand.size());
-            finished.push_back(r);
-        }
-        if (mr_n) {
-            std::vector<uint8_t> v(mr_n);
-            if (hipMemcpy(v.data(), s->d_mr_v.p, mr_n, hipMemcpyDeviceToHost) != hipSuccess) {
-                serve_fail_all(s, VOSS_EDEVICE, "MR verdicts");
-                return;
-            }
-            size_t o = 0;
-            for (auto& r : admitted_mr) {
-                std::memcpy(r->verdict.data(), v.data() + o, r->cand.size());
-                o += r->cand.size();
-            }
-        }
-        if (!finished.empty()) {
-            const double t = now_s();
-            std::lock_guard<std::mutex> lk(s->mu);
-            for (auto& r : finished) {
-                r->t_done = t;
-                r->done = true;
-                s->hbm_used -= r->hbm;
-                if (!r->is_mr) s->free_slots.push_back(r->slot);
-            }
-            s->cv_done.notify_all();
-        }
-    }
-}
-
-int serve_result(voss_server* s, ReqPtr r, voss_interval_result* out, uint8_t* verdict,
-                 double* latency_s) {
-    if (latency_s) *latency_s = r->t_done - r->t_submit;
-    if (r->status != VOSS_OK) return fail(r->status, s->errmsg.empty() ? "request failed" : s->errmsg);
-    if (r->is_mr) {
-        if (verdict && !r->verdict.empty()) std::memcpy(verdict, r->verdict.data(), r->verdict.size());
-    } else if (out) {
-        voss_large_gap* lg = out->large;
-        const uint64_t cap = out->large_capacity;
-        *out = r->res;
-        out->large = lg;
-        out->large_capacity = cap;
-        if (lg && cap) {
-            const size_t n = std::min<size_t>(r->large.size(), cap);
-            std::memcpy(lg, r->large.data(), n * sizeof(voss_large_gap));
-        }
-    }
-    s->all.erase(r->id);
-    return VOSS_OK;
-}
-
-}  // namespace
-
-extern "C" {
-
-int voss_serve_start(voss_ctx* c, const voss_serve_config* cfg, voss_server** out) {
-    if (!c || !out) return fail(VOSS_EINVAL, "NULL argument");
-    *out = nullptr;
-    VOSS_ADMIT(c);  // also refuses a context that already has a server
-    int rc = set_device(c);
-    if (rc) return rc;
-    voss_server* s = new voss_server();
-    s->c = c;
-    voss_serve_config d{64, 4096, 8ull << 30, 64, 1u << 22};
-    if (cfg) {
-        if (cfg->max_active) d.max_active = cfg->max_active;
-        if (cfg->windows_per_step) d.windows_per_step = cfg->windows_per_step;
-        if (cfg->hbm_budget_bytes) d.hbm_budget_bytes = cfg->hbm_budget_bytes;
-        if (cfg->large_capacity) d.large_capacity = cfg->large_capacity;
-        if (cfg->mr_per_step) d.mr_per_step = cfg->mr_per_step;
-    }
-    s->cfg = d;
-    hipError_t e = s->d_acc.reserve(d.max_active);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->mr_stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete s;
-        return hip_fail(e, "serve setup");
-    }
-    for (int i = (int)d.max_active - 1; i >= 0; i--) s->free_slots.push_back(i);
-    c->served = true;  // direct calls on c now return VOSS_EBUSY
-    s->th = std::thread(serve_loop, s);
-    *out = s;
-    return VOSS_OK;
-}
-
-int voss_serve_submit_interval(voss_server* s, uint64_t lo, uint64_t hi, uint64_t* id) {
-    if (!s || !id) return fail(VOSS_EINVAL, "NULL argument");
-    if (hi < lo || hi > (1ull << 62)) return fail(VOSS_EINVAL, "bad interval");
-    auto r = std::make_shared<ServeReq>();
-    r->lo = lo;
-    r->hi = hi;
-    r->t_submit = now_s();
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->err) return fail(s->err, s->errmsg);
-    if (s->stopping) return fail(VOSS_EBUSY, "server is stopping");
-    r->id = s->next_id++;
-    s->queue.push_back(r);
-    s->all[r->id] = r;
-    *id = r->id;
-    s->cv_work.notify_one();
-    return VOSS_OK;
-}
-
-int voss_serve_submit_mr(voss_server* s, const uint64_t* n, size_t count, uint64_t* id) {
-    if (!s || !id || (count && !n)) return fail(VOSS_EINVAL, "NULL argument");
-    if (count >= 0xFFFFFFFFull) return fail(VOSS_EINVAL, "count must be below 2^32");
-    auto r = std::make_shared<ServeReq>();
-    r->is_mr = true;
-    r->cand.assign(n, n + count);
-    r->t_submit = now_s();
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->err) return fail(s->err, s->errmsg);
-    if (s->stopping) return fail(VOSS_EBUSY, "server is stopping");
-    r->id = s->next_id++;
-    s->queue.push_back(r);
-    s->all[r->id] = r;
-    *id = r->id;
-    s->cv_work.notify_one();
-    return VOSS_OK;
-}
-
-int voss_serve_poll(voss_server* s, uint64_t id, int* done, voss_interval_result* out,
-                    uint8_t* verdict, double* latency_s) {
-    if (!s || !done) return fail(VOSS_EINVAL, "NULL argument");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->all.find(id);
-    if (it == s->all.end()) return fail(VOSS_ENOTFOUND, "unknown request id");
-    *done = it->second->done ? 1 : 0;
-    if (!*done) return VOSS_OK;
-    return serve_result(s, it->second, out, verdict, latency_s);
-}
-
-int voss_serve_wait(voss_server* s, uint64_t id, voss_interval_result* out, uint8_t* verdict,
-                    double* latency_s) {
-    if (!s) return fail(VOSS_EINVAL, "NULL argument");
-    std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->all.find(id);
-    if (it == s->all.end()) return fail(VOSS_ENOTFOUND, "unknown request id");
-    ReqPtr r = it->second;
-    s->cv_done.wait(lk, [&] { return r->done; });
-    return serve_result(s, r, out, verdict, latency_s);
-}
-
-int voss_serve_stop(voss_server* s) {
-    if (!s) return fail(VOSS_EINVAL, "NULL argument");
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->stopping = true;
-        s->cv_work.notify_all();
-    }
-    if (s->th.joinable()) s->th.join();
-    const int err = s->err;
-    Admit admit_(nullptr, true);
-    s->c->served = false;
-    (void)hipSetDevice(s->c->device);
-    if (s->mr_stream) (void)hipStreamDestroy(s->mr_stream);
-    s->d_acc.release();
-    s->h_acc.release();
-    s->d_mr_n.release();
-    s->d_mr_v.release();
-    s->d_surv.release();
-    s->d_nsurv.release();
-    delete s;
-    return err;
 }
 
 }  // extern "C"

@@ -135,7 +135,8 @@ EXPORTS = [
     "voss_run_interval", "voss_run_batch", "voss_run_batch_plain", "voss_mr_batch",
     "voss_mr_batch_device", "voss_mr_batch_device_noalloc", "voss_verify_gaps", "voss_device_alloc", "voss_device_free",
     "voss_memcpy_h2d", "voss_memcpy_d2h", "voss_serve_start", "voss_serve_submit_interval",
-    "voss_serve_submit_mr", "voss_serve_poll", "voss_serve_wait", "voss_serve_stop",
+    "voss_serve_submit_mr", "voss_serve_poll", "voss_serve_wait", "voss_serve_stats",
+    "voss_serve_stop",
     "voss_set_graph_mode", "voss_batch_prepare", "voss_batch_run", "voss_batch_free",
     "voss_batch_add_mr",
 ]
@@ -187,6 +188,8 @@ def lib():
         L.voss_serve_poll.argtypes = [vp, u64, P(i), vp, vp, P(ctypes.c_double)]
         L.voss_serve_wait.argtypes = [vp, u64, vp, vp, P(ctypes.c_double)]
         L.voss_serve_stop.argtypes = [vp]
+        L.voss_serve_stats.argtypes = [vp, P(u64), P(u64), P(ctypes.c_double),
+                                       P(ctypes.c_double)]
         for name in EXPORTS:
             if name not in ("voss_destroy", "voss_last_error", "voss_batch_free"):
                 getattr(L, name).restype = ctypes.c_int
@@ -535,6 +538,15 @@ class Server:
         if not done.value:
             return None
         return self._conv(rid, r, v), lat.value
+
+    def stats(self):
+        """{steps, windows, device_busy_s, wall_s} of the running server."""
+        st, win = ctypes.c_uint64(), ctypes.c_uint64()
+        busy, wall = ctypes.c_double(), ctypes.c_double()
+        _check(lib().voss_serve_stats(self._h, ctypes.byref(st), ctypes.byref(win),
+                                      ctypes.byref(busy), ctypes.byref(wall)))
+        return {"steps": st.value, "windows": win.value, "device_busy_s": busy.value,
+                "wall_s": wall.value}
 
     def stop(self):
         if self._h:
