@@ -403,9 +403,13 @@ def serve_leg(eng, W, n_req=300, n_cal=80, loads=(0.7, 1.3), windows_per_step=0)
     the plain path (which runs one request at a time in arrival order, as
     fast as it can): below capacity both paths keep up and latency shows
     the difference; above it the plain path's queue grows while continuous
-    batching overlaps requests on the device.  Latency = completion -
+    batching overlaps requests on the device.  Both servers run: the
+    device-resident loop (the default) and the host-stepped scheduler.
+    Every served result is compared with the same request run alone (all
+    fields of an interval, every MR verdict).  Latency = completion -
     arrival; req/s = requests / (last completion - first arrival)."""
     import threading
+    import vossgpu
     reqs = W.serve_requests(n_req + n_cal, 1.0)
     rng = np.random.default_rng(W.SEED0 + 4)
     payload = {}
@@ -416,21 +420,31 @@ def serve_leg(eng, W, n_req=300, n_cal=80, loads=(0.7, 1.3), windows_per_step=0)
     def run_alone(k):
         _, kind, p = reqs[k]
         if kind == "interval":
-            return eng.run_batch([p[0]], [p[1]], large_cap=64)[0][0]["total_primes"]
-        return int(eng.mr(payload[k]).sum())
+            return eng.run_batch([p[0]], [p[1]], large_cap=64)[0][0]
+        return eng.mr(payload[k])
 
+    def same(a, b):
+        if isinstance(a, np.ndarray):
+            return bool(np.array_equal(a, b))
+        return all(np.array_equal(a[f], b[f]) if isinstance(a[f], np.ndarray) else a[f] == b[f]
+                   for f in ("total_primes", "first_prime", "last_prime", "c1", "c3",
+                             "large_count", "gaps", "modq_gap", "modq", "large"))
+
+    modes = {"served": vossgpu.SERVE_DEVICE_LOOP, "served_host_steps": vossgpu.SERVE_HOST_STEPS}
     eng.run_interval(10**16, 10**16 + 10**6)  # base primes to 1e8 outside the timing
     t = time.perf_counter()
     for k in range(n_req, n_req + n_cal):  # calibration requests (not in the stream)
         run_alone(k)
     cap = n_cal / (time.perf_counter() - t)
-    # the calibration requests once through a server too, so that both paths
-    # start with warm buffers (the server's are kept by the context)
-    with eng.serve(max_active=64, windows_per_step=windows_per_step, large_capacity=64) as srv:
-        warm = [srv.submit_interval(*reqs[k][2]) if reqs[k][1] == "interval" else
-                srv.submit_mr(payload[k]) for k in range(n_req, n_req + n_cal)]
-        for rid in warm:
-            srv.wait(rid)
+    # the calibration requests once through each server too, so that every
+    # path starts with warm buffers (the servers' are kept by the context)
+    for mode in modes.values():
+        with eng.serve(max_active=64, windows_per_step=windows_per_step, large_capacity=64,
+                       mode=mode) as srv:
+            warm = [srv.submit_interval(*reqs[k][2]) if reqs[k][1] == "interval" else
+                    srv.submit_mr(payload[k]) for k in range(n_req, n_req + n_cal)]
+            for rid in warm:
+                srv.wait(rid)
 
     def pct(x, q):
         return float(np.percentile(np.array(x) * 1e3, q))
@@ -442,6 +456,7 @@ def serve_leg(eng, W, n_req=300, n_cal=80, loads=(0.7, 1.3), windows_per_step=0)
     out = {"requests": n_req, "plain_capacity_req_per_s": cap,
            "interval_numbers": sum(p[1] - p[0] for _, kind, p in reqs[:n_req] if kind == "interval"),
            "mr_tests": sum(p for _, kind, p in reqs[:n_req] if kind == "mr"), "loads": {}}
+    alone = {}
     for load in loads:
         lam = load * cap
         arrivals = [a / lam for a, _, _ in reqs[:n_req]]  # the stream at rate lam
@@ -452,35 +467,41 @@ def serve_leg(eng, W, n_req=300, n_cal=80, loads=(0.7, 1.3), windows_per_step=0)
             now = time.perf_counter() - t0
             if now < arrivals[k]:
                 time.sleep(arrivals[k] - now)
-            run_alone(k)
+            alone[k] = run_alone(k)
             lat_plain.append(time.perf_counter() - t0 - arrivals[k])
         span_plain = time.perf_counter() - t0
+        row = {"rate_req_per_s": lam, "plain": summary(lat_plain, span_plain)}
         # continuous batching
-        ids = [None] * n_req
-        with eng.serve(max_active=64, windows_per_step=windows_per_step,
-                       large_capacity=64) as srv:
-            t0 = time.perf_counter()
+        for name, mode in modes.items():
+            ids = [None] * n_req
+            with eng.serve(max_active=64, windows_per_step=windows_per_step,
+                           large_capacity=64, mode=mode) as srv:
+                t0 = time.perf_counter()
 
-            def submitter():
-                for k in range(n_req):
-                    now = time.perf_counter() - t0
-                    if now < arrivals[k]:
-                        time.sleep(arrivals[k] - now)
-                    _, kind, p = reqs[k]
-                    ids[k] = srv.submit_interval(*p) if kind == "interval" else \
-                        srv.submit_mr(payload[k])
+                def submitter():
+                    for k in range(n_req):
+                        now = time.perf_counter() - t0
+                        if now < arrivals[k]:
+                            time.sleep(arrivals[k] - now)
+                        _, kind, p = reqs[k]
+                        ids[k] = srv.submit_interval(*p) if kind == "interval" else \
+                            srv.submit_mr(payload[k])
 
-            th = threading.Thread(target=submitter)
-            th.start()
-            th.join()
-            lat_srv = [srv.wait(rid)[1] for rid in ids]
-            span_srv = time.perf_counter() - t0
-            st = srv.stats()
-        served = summary(lat_srv, span_srv)
-        served.update({"steps": st["steps"], "windows": st["windows"],
-                       "device_busy_frac": st["device_busy_s"] / span_srv})
-        out["loads"][f"{load:g}x"] = {"rate_req_per_s": lam, "served": served,
-                                      "plain": summary(lat_plain, span_plain)}
+                th = threading.Thread(target=submitter)
+                th.start()
+                th.join()
+                got = [srv.wait(rid) for rid in ids]
+                span_srv = time.perf_counter() - t0
+                st = srv.stats()
+                units = srv.unit_stats() if mode == vossgpu.SERVE_DEVICE_LOOP else None
+            served = summary([g[1] for g in got], span_srv)
+            served.update({"steps": st["steps"], "windows": st["windows"],
+                           "device_busy_frac": st["device_busy_s"] / span_srv,
+                           "equal_to_alone": all(same(alone[k], got[k][0]) for k in range(n_req))})
+            if units:
+                served["units"] = {k: {"count": n, "workgroup_s": t} for k, (n, t) in units.items()}
+            row[name] = served
+        out["loads"][f"{load:g}x"] = row
     return out
 
 

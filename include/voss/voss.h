@@ -275,23 +275,35 @@ int voss_verify_gaps(voss_ctx* ctx, const uint64_t* p_after,
 
 /* Continuous-batching server (no reference counterpart: the reference runs
  * one range per process).  A scheduler thread owns the context while the
- * server runs.  It admits queued requests while request slots and the HBM
- * budget allow.  Interval requests keep their accumulators in HBM slots
- * until they complete; each sieve step is made of up to windows_per_step
- * LDS windows taken round-robin from all active interval requests, and
- * steps are enqueued back to back (the host prepares the next step while
- * the device runs the current one, and collects a step's results after its
- * event).  Verify batches run on a second stream beside the sieve steps.
+ * server runs and admits queued requests, in arrival order, while request
+ * slots and the HBM budget allow.  Two schedulers:
+ *   VOSS_SERVE_DEVICE_LOOP (default)  a persistent kernel, one workgroup per
+ *     CU, claims units of work (an interval's bucket tasks and LDS windows,
+ *     chunks of a verify batch) from a ring in pinned host memory that the
+ *     scheduler appends admitted requests to; a request's sums stay in its
+ *     HBM slot, and the workgroup that completes its last unit joins the
+ *     pairs across its windows and writes the result to the request's
+ *     mailbox in pinned memory.  The scheduler never waits for the device
+ *     while requests run.  The loop sieves 131072-byte windows (the
+ *     context's tuning says so while the server runs).
+ *   VOSS_SERVE_HOST_STEPS  each sieve step is up to windows_per_step LDS
+ *     windows taken round-robin from all active interval requests, enqueued
+ *     back to back and collected after their events; verify batches run on
+ *     a second stream beside the steps.
  * Results are exactly those of voss_run_batch / voss_mr_batch on the same
  * request. */
 typedef struct voss_server voss_server;
 
+#define VOSS_SERVE_DEVICE_LOOP 0
+#define VOSS_SERVE_HOST_STEPS 1
 typedef struct {
     uint32_t max_active;       /* interval requests resident at once (default 64) */
-    uint32_t windows_per_step; /* LDS windows per sieve round (default 4096) */
-    uint64_t hbm_budget_bytes; /* admission budget for per-request device state */
+    uint32_t windows_per_step; /* host steps: LDS windows per sieve round (default 4096) */
+    uint64_t hbm_budget_bytes; /* admission budget for per-request device state
+                                  (default 32 GiB) */
     uint32_t large_capacity;   /* large gaps kept per interval request (default 64) */
-    uint32_t mr_per_step;      /* candidates per Miller-Rabin launch (default 2^22) */
+    uint32_t mr_per_step;      /* host steps: candidates per Miller-Rabin launch (default 2^22) */
+    uint32_t mode;             /* VOSS_SERVE_DEVICE_LOOP or VOSS_SERVE_HOST_STEPS */
 } voss_serve_config;
 
 int voss_serve_start(voss_ctx* ctx, const voss_serve_config* cfg, voss_server** out);
@@ -305,11 +317,18 @@ int voss_serve_poll(voss_server* s, uint64_t id, int* done, voss_interval_result
 int voss_serve_wait(voss_server* s, uint64_t id, voss_interval_result* out,
                     uint8_t* verdict, double* latency_s);
 /* Counters of a running server (no reference counterpart): sieve steps
- * and LDS windows enqueued so far, the device time of the steps collected so
- * far (HIP events around each step), and the wall time since
- * voss_serve_start.  Any pointer may be NULL. */
+ * enqueued so far (device loop: launches of the loop kernel), LDS windows
+ * enqueued so far (restarts included), the device time so far (host steps:
+ * HIP events around each collected step; device loop: the workgroups' time
+ * on units divided by the number of CUs, from the device wall clock), and
+ * the wall time since voss_serve_start.  Any pointer may be NULL. */
 int voss_serve_stats(voss_server* s, uint64_t* steps, uint64_t* windows, double* device_busy_s,
                      double* wall_s);
+/* Device loop only (no reference counterpart): per unit kind (0 windows,
+ * 1 bucket tasks, 2 Miller-Rabin chunks) the units run so far and the
+ * workgroup-seconds they took.  units and busy_s hold 3 entries each (zeros
+ * under the host-stepped scheduler). */
+int voss_serve_unit_stats(voss_server* s, uint64_t* units, double* busy_s);
 /* Finishes every submitted request, stops the scheduler, frees the server. */
 int voss_serve_stop(voss_server* s);
 

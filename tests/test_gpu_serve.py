@@ -1,11 +1,16 @@
-"""Continuous-batching server: requests that overlap on the device give
-exactly the results of the same request run alone (tolerance 0)."""
+"""Continuous-batching server, both schedulers (the device-resident loop and
+the host-stepped one): requests that overlap on the device give exactly the
+results of the same request run alone (tolerance 0)."""
 import threading
 
 import numpy as np
 import pytest
 
+import vossgpu
+
 pytestmark = pytest.mark.gpu
+
+MODES = [vossgpu.SERVE_DEVICE_LOOP, vossgpu.SERVE_HOST_STEPS]
 
 LARGE = ("total_primes", "first_prime", "last_prime", "c1", "c3", "large_count")
 
@@ -32,11 +37,13 @@ def workload():
     return iv, mr
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("cfg", [dict(max_active=4, windows_per_step=24),
                                  dict(max_active=64, windows_per_step=4096),
                                  dict(max_active=3, windows_per_step=7, hbm_budget_bytes=40000)])
-def test_served_equals_alone(engine, cfg):
+def test_served_equals_alone(engine, cfg, mode):
     iv, mr = workload()
+    cfg = dict(cfg, mode=mode)
     print(f"[gpu] serve {cfg}: alone", flush=True)
     alone_iv = [engine.run_batch([a], [b], large_cap=64)[0][0] for a, b in iv]
     alone_mr = [engine.mr(x) for x in mr]
@@ -68,13 +75,14 @@ def test_served_equals_alone(engine, cfg):
 
 
 def test_served_step_counters(engine):
-    """The server's counters: every window of every interval request is
-    enqueued exactly once (no restarts here), in at least as many steps as
-    the step size requires, and the steps' device time is within the wall
-    time."""
+    """The host-stepped server's counters: every window of every interval
+    request is enqueued exactly once (no restarts here), in at least as many
+    steps as the step size requires, and the steps' device time is within
+    the wall time."""
     iv, _ = workload()
     alone_windows = sum(engine.run_batch([a], [b])[1]["windows"] for a, b in iv)
-    with engine.serve(max_active=8, windows_per_step=64, large_capacity=64) as srv:
+    with engine.serve(max_active=8, windows_per_step=64, large_capacity=64,
+                      mode=vossgpu.SERVE_HOST_STEPS) as srv:
         ids = [srv.submit_interval(a, b) for a, b in iv]
         for rid in ids:
             srv.wait(rid)
@@ -84,7 +92,24 @@ def test_served_step_counters(engine):
     assert 0 < st["device_busy_s"] <= st["wall_s"]
 
 
-def test_served_bucket_overflow_restarts_exactly(engine):
+def test_loop_counters(engine):
+    """The device loop's counters: every window published once (no
+    restarts here), at least one launch of the loop kernel, its resident
+    time within the wall time."""
+    iv, _ = workload()
+    alone_windows = sum(engine.run_batch([a], [b])[1]["windows"] for a, b in iv)
+    with engine.serve(max_active=8, large_capacity=64) as srv:
+        ids = [srv.submit_interval(a, b) for a, b in iv]
+        for rid in ids:
+            srv.wait(rid)
+        st = srv.stats()
+    assert st["windows"] == alone_windows
+    assert st["steps"] >= 1
+    assert 0 < st["device_busy_s"] <= st["wall_s"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_served_bucket_overflow_restarts_exactly(engine, mode):
     """Bucket rooms at 50 % of their estimate: the first steps with bucketed
     windows overflow, their requests start over with exact rooms, and every
     result still equals the request run alone."""
@@ -95,7 +120,7 @@ def test_served_bucket_overflow_restarts_exactly(engine):
     engine.set_tuning(bucket_room_pct=50, bucket_limit=1 << 17, bucket_min_windows=2)
     try:
         print("[gpu] serve with undersized bucket rooms", flush=True)
-        with engine.serve(max_active=4, windows_per_step=16, large_capacity=64) as srv:
+        with engine.serve(max_active=4, windows_per_step=16, large_capacity=64, mode=mode) as srv:
             got = [srv.wait(srv.submit_interval(a, b))[0] for a, b in iv]
             st = srv.stats()
     finally:

@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
-KERNEL_SOURCES = ["sieve.hip", "mr.hip", "api.hip", "serve.hip"]
+KERNEL_SOURCES = ["sieve.hip", "mr.hip", "api.hip", "serve.hip", "serve_steps.hip", "serve_kernel.hip"]
 
 
 def _run(cmd):

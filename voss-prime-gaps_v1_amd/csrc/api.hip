@@ -459,7 +459,7 @@ double bucket_task_cost(const voss_ctx* c, const BucketTask& T, uint32_t W) {
            210.0 * span * (c->inv_prefix[T.i1] - c->inv_prefix[T.iw]);
 }
 
-void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
+void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds, double min_share) {
     const uint32_t W = c->tune.window_bytes;
     const uint32_t bl = c->tune.bucket_limit;
     const size_t ibl = (size_t)(std::lower_bound(c->primes.begin(), c->primes.end(), bl) -
@@ -569,7 +569,7 @@ void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds) {
             // prime chunks of about a quarter of that share (shared tasks)
             double total = 0;
             for (const auto& w : work) total += w.first;
-            const double share = total / (2.0 * std::max(c->num_cus, 1));  // two workgroups per CU
+            const double share = std::max(min_share, total / (2.0 * std::max(c->num_cus, 1)));  // two workgroups per CU
             std::vector<std::pair<double, BucketTask>> split;
             for (const auto& w : work) {
                 if (w.first <= share) {
@@ -747,7 +747,6 @@ int voss_create(int device, voss_ctx** out) {
         if (e == hipSuccess) e = hipEventCreate(&c->ev0);
         if (e == hipSuccess) e = hipEventCreate(&c->ev1);
         if (e == hipSuccess) e = hipEventCreate(&c->ev2);
-        if (e == hipSuccess) e = upload_sieve_constants(device);
         if (e != hipSuccess) rc = hip_fail(e, "context setup");
         if (rc == VOSS_OK && (e = mr_scratch_address(device, &c->mr_scratch)) != hipSuccess)
             rc = hip_fail(e, "hipGetSymbolAddress(scratch)");

@@ -510,7 +510,10 @@ struct Round {
 // span/p per walked prime).
 double bucket_task_cost(const voss_ctx* c, const BucketTask& T, uint32_t W);
 
-void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds);
+// min_share: a tile is split into shared tasks only beyond this estimated
+// cost (and beyond a fair share of the whole plan).
+void make_rounds(const voss_ctx* c, const Plan& P, std::vector<Round>& rounds,
+                 double min_share = 0.0);
 
 int log2_window(const voss_ctx* c);
 

@@ -35,7 +35,6 @@ struct SieveArgs {
                          // window blockIdx.x, then gridDim.x + atomicAdd
 };
 
-hipError_t upload_sieve_constants(int device);
 hipError_t launch_init_acc(JobAcc* acc, uint32_t njobs, hipStream_t s);
 hipError_t launch_sieve(const SieveArgs& a, int window_bytes, uint32_t grid,
                         hipStream_t s);

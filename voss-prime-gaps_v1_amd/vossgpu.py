@@ -118,7 +118,11 @@ def interval_to_dict(r):
 class ServeConfig(ctypes.Structure):
     _fields_ = [("max_active", ctypes.c_uint32), ("windows_per_step", ctypes.c_uint32),
                 ("hbm_budget_bytes", ctypes.c_uint64), ("large_capacity", ctypes.c_uint32),
-                ("mr_per_step", ctypes.c_uint32)]
+                ("mr_per_step", ctypes.c_uint32), ("mode", ctypes.c_uint32)]
+
+
+SERVE_DEVICE_LOOP = 0  # persistent loop kernel fed from a pinned ring (default)
+SERVE_HOST_STEPS = 1   # host-planned sieve steps enqueued back to back
 
 
 class Tuning(ctypes.Structure):
@@ -136,7 +140,7 @@ EXPORTS = [
     "voss_mr_batch_device", "voss_mr_batch_device_noalloc", "voss_verify_gaps", "voss_device_alloc", "voss_device_free",
     "voss_memcpy_h2d", "voss_memcpy_d2h", "voss_serve_start", "voss_serve_submit_interval",
     "voss_serve_submit_mr", "voss_serve_poll", "voss_serve_wait", "voss_serve_stats",
-    "voss_serve_stop",
+    "voss_serve_unit_stats", "voss_serve_stop",
     "voss_set_graph_mode", "voss_batch_prepare", "voss_batch_run", "voss_batch_free",
     "voss_batch_add_mr",
 ]
@@ -188,6 +192,7 @@ def lib():
         L.voss_serve_poll.argtypes = [vp, u64, P(i), vp, vp, P(ctypes.c_double)]
         L.voss_serve_wait.argtypes = [vp, u64, vp, vp, P(ctypes.c_double)]
         L.voss_serve_stop.argtypes = [vp]
+        L.voss_serve_unit_stats.argtypes = [vp, P(u64), P(ctypes.c_double)]
         L.voss_serve_stats.argtypes = [vp, P(u64), P(u64), P(ctypes.c_double),
                                        P(ctypes.c_double)]
         for name in EXPORTS:
@@ -430,11 +435,11 @@ class Engine:
         return out, times
 
     def serve(self, max_active=0, windows_per_step=0, hbm_budget_bytes=0, large_capacity=0,
-              mr_per_step=0):
+              mr_per_step=0, mode=SERVE_DEVICE_LOOP):
         """A continuous-batching Server on this engine (which it owns until
-        the server stops)."""
+        the server stops); mode SERVE_DEVICE_LOOP or SERVE_HOST_STEPS."""
         return Server(self, ServeConfig(max_active, windows_per_step, hbm_budget_bytes,
-                                        large_capacity, mr_per_step))
+                                        large_capacity, mr_per_step, mode))
 
     def verify_gaps(self, p_after, gaps):
         pa = np.ascontiguousarray(p_after, dtype=np.uint64)
@@ -547,6 +552,14 @@ class Server:
                                       ctypes.byref(busy), ctypes.byref(wall)))
         return {"steps": st.value, "windows": win.value, "device_busy_s": busy.value,
                 "wall_s": wall.value}
+
+    def unit_stats(self):
+        """Device loop: {kind: (units, workgroup_seconds)} for windows,
+        bucket tasks and Miller-Rabin chunks."""
+        n = (ctypes.c_uint64 * 3)()
+        b = (ctypes.c_double * 3)()
+        _check(lib().voss_serve_unit_stats(self._h, n, b))
+        return {k: (int(n[i]), float(b[i])) for i, k in enumerate(("window", "bucket", "mr"))}
 
     def stop(self):
         if self._h:
