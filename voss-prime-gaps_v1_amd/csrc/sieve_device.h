@@ -833,7 +833,15 @@ __device__ __forceinline__ void walk_hits(uint64_t w0, const PrimeRec& pr, uint3
         const uint32_t r = mod_u64(X, p, pr.magic);
         x = r ? p - r : 0u;
     }
+    // the tile holds 30*span < 2^30 numbers; most walk-mode primes of a
+    // short tile have no multiple in it, and one beyond 30*span at most one
+    const uint32_t L = 30u * span;
+    if (x >= L) return;
     uint32_t b = x / 30u, rr = x - 30u * b;
+    if (p >= L) {
+        if ((kCoprime30 >> rr) & 1u) f(b, wheel_bit_of_residue(rr));
+        return;
+    }
     const uint32_t pd = p / 30u, pm = p - 30u * pd;
     while (b < span) {
         if ((kCoprime30 >> rr) & 1u) f(b, wheel_bit_of_residue(rr));
