@@ -27,48 +27,6 @@
 namespace voss {
 namespace serving {
 
-bool RangeAlloc::alloc(uint64_t n, uint64_t align, uint64_t* off) {
-    if (n == 0) {
-        *off = 0;
-        return true;
-    }
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-        const uint64_t o = (it->first + align - 1) / align * align;
-        const uint64_t end = it->first + it->second;
-        if (o + n > end) continue;
-        const uint64_t f0 = it->first;
-        free_.erase(it);
-        if (o > f0) free_[f0] = o - f0;
-        if (o + n < end) free_[o + n] = end - (o + n);
-        *off = o;
-        return true;
-    }
-    return false;
-}
-
-void RangeAlloc::release(uint64_t off, uint64_t n) {
-    if (n == 0) return;
-    auto it = free_.emplace(off, n).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-        it->second += nx->second;
-        free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-        auto pv = std::prev(it);
-        if (pv->first + pv->second == it->first) {
-            pv->second += it->second;
-            free_.erase(it);
-        }
-    }
-}
-
-uint64_t RangeAlloc::largest() const {
-    uint64_t m = 0;
-    for (const auto& f : free_) m = std::max(m, f.second);
-    return m;
-}
-
 hipError_t PinnedBuf::reserve(size_t n) {
     if (n <= bytes && p) return hipSuccess;
     release();

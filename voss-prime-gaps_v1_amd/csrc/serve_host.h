@@ -11,6 +11,7 @@
 #include <unordered_map>
 
 #include "engine.h"
+#include "range_alloc.h"
 #include "serve_launch.h"
 
 namespace voss {
@@ -83,20 +84,6 @@ struct MrLaunch {
     std::vector<ReqPtr> reqs;
     uint64_t n = 0;
     bool busy = false;
-};
-
-// First-fit allocator of ranges of [0, cap).
-struct RangeAlloc {
-    uint64_t cap = 0;
-    std::map<uint64_t, uint64_t> free_;  // offset -> length
-    void reset(uint64_t c) {
-        cap = c;
-        free_.clear();
-        if (c) free_[0] = c;
-    }
-    bool alloc(uint64_t n, uint64_t align, uint64_t* off);
-    void release(uint64_t off, uint64_t n);
-    uint64_t largest() const;
 };
 
 // Pinned host memory that the device reads and writes while the loop runs
