@@ -397,7 +397,7 @@ def batch_with_mr(eng, lo, hi, cand, rb, same, replay_s, out):
 
 
 def serve_leg(eng, W, n_req=300, n_cal=80, loads=(0.7, 1.3), windows_per_step=0):
-    """Config 5: a Poisson stream (half interval requests with start in
+    """Serving: a Poisson stream (half interval requests with start in
     [1e10, 1e16) and length log-uniform in [1e5, 1e10], half MR batches of
     1e3..1e6 candidates), offered at each of `loads` times the capacity of
     the plain path (which runs one request at a time in arrival order, as

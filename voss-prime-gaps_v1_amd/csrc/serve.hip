@@ -133,9 +133,18 @@ constexpr uint64_t kTileMin = 1ull << 14;
 constexpr uint32_t kIdleLimit = 16384;           // empty polls (a few us each) before exit
 constexpr uint32_t kDepLimit = 1u << 22;         // polls of a bucket dependency (seconds)
 constexpr uint32_t kLoopWindow = 131072;         // the loop kernel's window
-constexpr uint32_t kAheadPerWorkgroup = 2;       // units published ahead of the device
+constexpr uint32_t kAheadPerWorkgroup = 1;       // units published ahead of the device
 constexpr uint32_t kBurst = 8;                   // units of one request per turn
-constexpr double kMinTaskCost = 1.5e7;           // ~100 us of a workgroup (bucket_task_cost units)
+// A tile is split into bucket tasks of at least this estimated cost
+// (bucket_task_cost units): ~0.35 ms of a workgroup when the device has
+// little queued work, so one request spreads over many workgroups, and
+// ~1.4 ms when units are piling up (a split tile's tasks count their hits
+// before they write them, so fewer, larger tasks do less work: measured on
+// a Poisson stream of intervals in [1e10, 1e16) of lengths 1e5..1e10 mixed
+// with Miller-Rabin batches, the loop's bucket work fell from 20 to 11
+// workgroup-seconds between the two).
+constexpr double kMinTaskCostIdle = 1.5e7;
+constexpr double kMinTaskCostBusy = 6e7;
 constexpr uint32_t kMaxBypass = 256;             // admissions that may pass a waiting request
 
 template <typename T>
@@ -150,7 +159,6 @@ struct Loop {
     RangeAlloc ent, rec, cur, tile, host;
     std::vector<ReqPtr> live;  // admitted, not yet complete
     std::vector<ReqPtr> feed;  // admitted, with units not yet published
-    size_t rr = 0;             // the request whose turn it is in feed
     std::vector<int> mr_free;
     std::vector<uint32_t> slot_gen, mr_gen;
     uint64_t tail = 0;
@@ -276,6 +284,14 @@ struct Loop {
         return m;
     }
 
+    // Units of the admitted requests that the device has not claimed yet
+    // (published ahead, or still to be published).
+    uint64_t queued_units() const {
+        uint64_t n = tail - claimed();
+        for (const auto& r : feed) n += (r->units.size() - r->upub) + (r->nwin - r->wpub);
+        return n;
+    }
+
     // The next unit request r may publish now: a window whose tile's bucket
     // tasks are done (or that has none), else its next bucket task or MR
     // chunk.  false when it has none ready.
@@ -295,18 +311,27 @@ struct Loop {
     }
 
     // Keeps about kAheadPerWorkgroup units per workgroup published ahead of
-    // the device, taking up to kBurst units from each admitted request in
-    // turn.
+    // the device.  Each burst of up to kBurst units goes to the request that
+    // has had the fewest units published so far among those with a unit
+    // ready (least attained service: a short request that arrives behind a
+    // long one is fed first).  Measured on that mixed stream at 1.3x the
+    // plain path's capacity: p50 1.16 -> 0.92 ms and p99 16.9 -> 13.1 ms
+    // against two units ahead in plain round-robin, at the same rate.
     void publish_some() {
         if (feed.empty()) return;
         const uint64_t ahead = (uint64_t)kAheadPerWorkgroup * c->num_cus;
         const uint64_t cl = claimed();
         ServeUnit* ring = static_cast<ServeUnit*>(L.ring.p);
         const uint64_t t0 = tail;
-        size_t idle = 0;  // requests in a row that had nothing ready
-        while (!feed.empty() && tail - cl < ahead && idle < feed.size()) {
-            if (rr >= feed.size()) rr = 0;
-            ServeReq* r = feed[rr].get();
+        std::vector<uint8_t> blocked(feed.size(), 0);  // nothing ready in this call
+        while (tail - cl < ahead) {
+            size_t k = feed.size();
+            for (size_t j = 0; j < feed.size(); j++) {
+                if (blocked[j]) continue;
+                if (k == feed.size() || feed[j]->upub + feed[j]->wpub < feed[k]->upub + feed[k]->wpub) k = j;
+            }
+            if (k == feed.size()) break;
+            ServeReq* r = feed[k].get();
             uint32_t n = 0;
             ServeUnit u;
             while (n < kBurst && tail - cl < ahead && next_unit(r, &u)) {
@@ -314,11 +339,11 @@ struct Loop {
                 ring[(tail++) & (L.ring_cap - 1)] = u;
                 n++;
             }
-            idle = n ? 0 : idle + 1;
             if (r->upub == r->units.size() && (r->is_mr || r->wpub == r->nwin)) {
-                feed.erase(feed.begin() + (long)rr);
-            } else {
-                rr++;
+                feed.erase(feed.begin() + (long)k);
+                blocked.erase(blocked.begin() + (long)k);
+            } else if (n < kBurst && tail - cl < ahead) {
+                blocked[k] = 1;
             }
         }
         if (tail != t0) __atomic_store_n(&ctl()->tail, (unsigned long long)tail, __ATOMIC_RELEASE);
@@ -367,7 +392,8 @@ struct Loop {
         std::vector<Round> rounds;
         const uint32_t pct0 = c->tune.bucket_room_pct;
         if (r->room_pct) c->tune.bucket_room_pct = r->room_pct;
-        make_rounds(c, P, rounds, kMinTaskCost);
+        make_rounds(c, P, rounds,
+                    queued_units() > 4ull * (uint64_t)c->num_cus ? kMinTaskCostBusy : kMinTaskCostIdle);
         c->tune.bucket_room_pct = pct0;
         const bool bucketed = rounds.size() == 1 && !rounds[0].desc.empty() &&
                               rounds[0].room_total <= kEntriesMax / 2;
