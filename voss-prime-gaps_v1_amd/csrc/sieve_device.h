@@ -594,13 +594,22 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
     }
     const PairCtx pc{S.hist, S.lgt, io.acc, io.large, io.large_n, io.large_cap,
                      base_num, io.job, S.gmaybe};
+    // Each thread walks one run of consecutive words; the runs are in
+    // thread order.  Runs of `per` words (L4 / BLOCK) put the k-th word of
+    // every run of a wave in one LDS bank (per = 32 at full windows: all 64
+    // lanes of an in-order read on one bank).  Instead, the 32 threads of a
+    // group share the group's 32*per words in runs of per + 1 words (the
+    // last runs of a group are shorter, or empty), so at step k thread j of
+    // the group reads bank (j*(per+1) + k) mod 32: a different bank for each
+    // thread when per + 1 is odd.
     const uint32_t per = L4 / BLOCK;
-    const uint32_t i0 = (uint32_t)tid * per;
-    const uint32_t iend = i0 + per;
+    const uint32_t gbase = ((uint32_t)tid >> 5) * 32u * per, gend = gbase + 32u * per;
+    const uint32_t i0 = min(gbase + ((uint32_t)tid & 31u) * (per + 1u), gend);
+    const uint32_t iend = min(i0 + per + 1u, gend);
     // pre-pass: clip the run to [n_lo, n_hi] (only runs at the window's
     // ends need it) and add its words into bit-sliced counters: bit j
     // of v[k] is bit k of the number of words with bit j set (a run has
-    // at most 32 words, so six planes hold every count)
+    // at most 33 words, so six planes hold every count)
     uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     auto add_word = [&](uint32_t w) {
         uint32_t c = w, t;
@@ -624,14 +633,7 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
             add_word(w);
         }
     } else {
-        // the order does not matter here: each lane starts at its own
-        // rotation of the run, so the 64 lanes of a wave (runs 32 words
-        // apart) read different banks instead of one
-        uint32_t i = i0 + (uint32_t)tid % per;
-        for (uint32_t q = 0; q < per; q++) {
-            add_word(bm[i]);
-            if (++i == iend) i = i0;
-        }
+        for (uint32_t i = i0; i < iend; i++) add_word(bm[i]);
     }
     // counts from the planes: all primes, p % 4 == 1 (bytes 0 and 2 of a
     // word have even k: residues {1,13,17,29} there, {7,11,19,23} at odd
