@@ -303,6 +303,10 @@ struct alignas(16) SieveShared {
     // walk: how many nonzero wheel bytes of each value were seen (their
     // internal prime pairs go into mq at the flush)
     uint32_t bytes[EMIT ? 1 : 256];
+    // walk: per wheel byte value v, W30 of its lowest set bit (bits 0-4),
+    // W30 of its highest (5-9), 4 * the highest bit (10-14), and bit 15 set
+    // when v holds two primes or more
+    uint16_t blut[EMIT ? 1 : 256];
     uint32_t pat[kPatLds ? kPresieveWordsPadded : 1];
     uint32_t tfirst[BLOCK];
     uint32_t tlast[BLOCK];
@@ -335,8 +339,16 @@ __device__ __forceinline__ void sieve_init_lds(const SieveArgs& a, SieveShared<W
     const int tid = threadIdx.x;
     for (int i = tid; i < kHistBins; i += BLOCK) S.hist[i] = 0;
     for (int i = tid; i < 8 * 64; i += BLOCK) S.mq[i] = 0;
-    if constexpr (!EMIT)
-        for (int i = tid; i < 256; i += BLOCK) S.bytes[i] = 0;
+    if constexpr (!EMIT) {
+        for (int i = tid; i < 256; i += BLOCK) {
+            S.bytes[i] = 0;
+            const uint32_t v = (uint32_t)i;
+            const uint32_t lo = (uint32_t)__builtin_ctz(v | 0x100u), hi = 31u - (uint32_t)__builtin_clz(v | 1u);
+            S.blut[i] = v ? (uint16_t)(w30_of(lo) | (w30_of(hi) << 5) | ((4u * hi) << 10) |
+                                       ((v & (v - 1u)) ? 0x8000u : 0u))
+                          : (uint16_t)0;
+        }
+    }
     if constexpr (SieveShared<WIN, BLOCK, EMIT>::kPatLds)
         for (int i = tid; i < kPresieveWordsPadded; i += BLOCK) S.pat[i] = a.pat[i];
     for (int i = tid; i < kLgtCount; i += BLOCK) S.lgt[i] = a.lg_thresh[kLgtFirst + i];
@@ -726,21 +738,16 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
                 for (uint32_t j = 0; j < 4; j++) {
                     const uint32_t v = (cw >> (8u * j)) & 0xFFu;
                     if (v) {
-                        const uint32_t blo = (uint32_t)__builtin_ctz(v);
-                        const uint32_t bhi = 31u - (uint32_t)__builtin_clz(v);
+                        const uint32_t e = S.blut[v];
                         const uint32_t bb = wb + 30u * j;
-                        const uint32_t off_lo =
-                            bb + __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, blo | 0x0C0C0C00u);
-                        const uint32_t off_hi =
-                            bb + __builtin_amdgcn_perm(0x1D171311u, 0x0D0B0701u, bhi | 0x0C0C0C00u);
-                        const uint32_t g = off_lo - prevoff;  // even
+                        const uint32_t g = bb + (e & 31u) - prevoff;  // even
                         char* const at = g < 100u ? mqb + (16u * g + rowb)
                                                   : hb + 2u * min(g, 2u * kHistBins - 2u);
                         atomicAdd(reinterpret_cast<uint32_t*>(at), 1u);
                         if (g >= pc.g_maybe) pair_large(pc, prevoff, g);
-                        if (v & (v - 1u)) atomicAdd(&S.bytes[v], 1u);
-                        rowb = bhi << 2;
-                        prevoff = off_hi;
+                        if (e & 0x8000u) atomicAdd(&S.bytes[v], 1u);
+                        rowb = (e >> 10) & 0x1Cu;
+                        prevoff = bb + ((e >> 5) & 31u);
                     }
                 }
             }
