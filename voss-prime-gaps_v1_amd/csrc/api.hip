@@ -117,9 +117,13 @@ int ensure_primes(voss_ctx* c, uint64_t max_number) {
     }
     const size_t n = std::max<size_t>(c->primes.size(), 1);
     HIPCHK(c->d_primes.reserve(n));
-    if (!c->primes.empty())
+    HIPCHK(c->d_p32.reserve(n));
+    if (!c->primes.empty()) {
         HIPCHK(hipMemcpy(c->d_primes.p, rec.data(), rec.size() * sizeof(PrimeRec),
                          hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_p32.p, c->primes.data(), c->primes.size() * sizeof(uint32_t),
+                         hipMemcpyHostToDevice));
+    }
     c->d_nprimes = c->primes.size();
     return VOSS_OK;
 }
@@ -415,6 +419,7 @@ SieveArgs make_args(voss_ctx* c, ExecBufs& B, uint32_t ndesc, unsigned long long
     if (a.tier_a_end < a.tier_a_begin) a.tier_a_end = a.tier_a_begin;
     a.restore0 = c->restore0;
     a.primes = c->d_primes.p;
+    a.p32 = c->d_p32.p;
     const int block = sieve_block_threads((int)c->tune.window_bytes);
     a.pat = c->d_pat.p;
     for (int g = 0; g < kPresieveGroups; g++) {
@@ -768,6 +773,7 @@ void voss_destroy(voss_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->d_primes.release();
+    c->d_p32.release();
     c->d_pat.release();
     c->d_lg_thresh.release();
     c->eb.release();

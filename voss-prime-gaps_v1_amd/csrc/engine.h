@@ -300,6 +300,7 @@ struct voss_ctx {
     std::vector<double> inv_prefix;  // inv_prefix[i] = sum of 1/p over primes[0..i)
     uint64_t primes_limit = 0;
     DevBuf<PrimeRec> d_primes;
+    DevBuf<uint32_t> d_p32;  // the same primes as 32-bit values (bucket walk mode)
     size_t d_nprimes = 0;
 
     DevBuf<uint32_t> d_pat;
@@ -481,8 +482,12 @@ struct ExecTimes {
     uint32_t reruns = 0;   // re-executions after a bucket or large-list overflow
 };
 
-// Bucket entries per round (8 GiB of HBM); offsets stay 32-bit.
-constexpr uint64_t kBucketEntryBudget = 1ull << 31;
+// Bucket entries per round (32 GiB of HBM).  Bucket offsets are 64-bit; a
+// fill task's slots are 32-bit offsets from its first bucket.  Larger
+// rounds hold more tiles at once, so fewer of them are split into prime
+// chunks that count before they write (4096-interval batch, bucket fills:
+// 90.7 ms at 2^31 entries per round, 83.4 ms at 2^33).
+constexpr uint64_t kBucketEntryBudget = 1ull << 33;
 
 // Room for the entries of one bucket (a window of W bytes) fed by primes
 // with sum(1/p) = s.  A prime p has on average 8W/p multiples coprime to 30
@@ -535,7 +540,8 @@ struct Executor {
     bool exact = false;        // bucket rooms are the exact counts
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    const PrimeRec* graph_primes = nullptr;  // base-prime table the graph reads
+    const PrimeRec* graph_primes = nullptr;  // base-prime tables the graph reads
+    const uint32_t* graph_p32 = nullptr;
     bool copy_acc = true;      // read the accumulators back after a run
     uint64_t nlarge = 0;       // large-gap records of the last run (in B->h_large)
     // Miller-Rabin batches run beside the rounds on a second stream (forked
@@ -637,7 +643,8 @@ struct Executor {
             ent = std::max<uint64_t>(ent, R.room_total);
             nb_max = std::max<uint32_t>(nb_max, R.nbuckets);
         }
-        if (ent >= 0xFFFFFFFFull) return fail(VOSS_ENOMEM, "bucket round beyond 2^32 entries");
+        if (ent > 2 * kBucketEntryBudget)
+            return fail(VOSS_ENOMEM, "bucket round beyond its entry budget");
         drop_graph();
         HIPCHK(B->d_desc.reserve(nd));
         HIPCHK(B->d_rec.reserve(nd));
@@ -723,7 +730,7 @@ struct Executor {
             if (!R.tasks.empty()) {
                 HIPCHK(launch_fill_u32(task_counter(), 1, 0, s));
                 HIPCHK(launch_bucket_fill(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
-                                          c->d_primes.p, B->d_boff.p + R.boff_off,
+                                          c->d_primes.p, c->d_p32.p, B->d_boff.p + R.boff_off,
                                           B->d_bcur.p + R.bkt_off, B->d_bent.p,
                                           (uint32_t*)(B->d_flags.p + 1), task_counter(), log2w,
                                           bgrid, s));
@@ -775,7 +782,7 @@ struct Executor {
                 HIPCHK(launch_fill_u32(B->d_bcnt.p, R.nbuckets, 0, s));
                 HIPCHK(launch_fill_u32(task_counter(), 1, 0, s));
                 HIPCHK(launch_bucket_count(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
-                                           c->d_primes.p, B->d_bcnt.p, task_counter(), log2w,
+                                           c->d_primes.p, c->d_p32.p, B->d_bcnt.p, task_counter(), log2w,
                                            bgrid, s));
                 HIPCHK(hipMemcpyAsync(B->h_bcnt.p, B->d_bcnt.p, R.nbuckets * 4,
                                       hipMemcpyDeviceToHost, s));
@@ -788,7 +795,8 @@ struct Executor {
                 }
                 B->h_boff.p[R.boff_off + R.nbuckets] = o;
                 R.room_total = o;
-                if (o >= 0xFFFFFFFFull) return fail(VOSS_ENOMEM, "bucket round beyond 2^32 entries");
+                if (o > 2 * kBucketEntryBudget)
+                    return fail(VOSS_ENOMEM, "bucket round beyond its entry budget");
                 // the stream is idle here: growing the entry buffer is safe
                 HIPCHK(B->d_bent.reserve(std::max<uint64_t>(o, 1)));
                 HIPCHK(hipMemcpyAsync(B->d_boff.p + R.boff_off, B->h_boff.p + R.boff_off,
@@ -796,7 +804,7 @@ struct Executor {
                 HIPCHK(launch_fill_u32(B->d_bcur.p + R.bkt_off, R.nbuckets, 0, s));
                 HIPCHK(launch_fill_u32(task_counter(), 1, 0, s));
                 HIPCHK(launch_bucket_fill(B->d_btask.p + R.task_off, (uint32_t)R.tasks.size(),
-                                          c->d_primes.p, B->d_boff.p + R.boff_off,
+                                          c->d_primes.p, c->d_p32.p, B->d_boff.p + R.boff_off,
                                           B->d_bcur.p + R.bkt_off, B->d_bent.p,
                                           (uint32_t*)(B->d_flags.p + 1), task_counter(), log2w,
                                           bgrid, s));
@@ -826,6 +834,7 @@ struct Executor {
         graph = g;
         HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
         graph_primes = c->d_primes.p;
+        graph_p32 = c->d_p32.p;
         return VOSS_OK;
     }
 
@@ -836,7 +845,8 @@ struct Executor {
         t.rounds = (uint32_t)rounds.size();
         int rc;
         if ((rc = ensure_events(4 * rounds.size()))) return rc;
-        if (gexec && graph_primes != c->d_primes.p) drop_graph();  // base primes moved
+        if (gexec && (graph_primes != c->d_primes.p || graph_p32 != c->d_p32.p))
+            drop_graph();  // base primes moved
         bool breakdown = false;
         for (;;) {
             if (!staged && (rc = stage())) return rc;

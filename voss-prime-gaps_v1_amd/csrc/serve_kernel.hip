@@ -301,7 +301,7 @@ __device__ void serve_bucket(const SieveArgs& a, const ServeArgs& sv, ServeLds& 
     uint32_t* lim = S.bm + kBucketTileMax;
     // one workgroup per CU here (the sieve's LDS): eight records in flight
     // per thread make up for the occupancy of the batch form's fill kernel
-    bucket_task<true, true, 8>(T, a.primes, X.req.cursor, X.req.boff, sv.entries, &sv.slot[slot].overflow,
+    bucket_task<true, true, 8>(T, a.primes, a.p32, X.req.cursor, X.req.boff, sv.entries, &sv.slot[slot].overflow,
                       sv.log2w, cnt, lim);
     // the entries and cursors before the tile's count; the tile's last task
     // tells the host, which publishes the tile's windows after that

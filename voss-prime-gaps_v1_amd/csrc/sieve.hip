@@ -187,6 +187,7 @@ template <bool FILL>
 __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restrict__ tasks,
                                                       uint32_t ntask,
                                                       const PrimeRec* __restrict__ primes,
+                                                      const uint32_t* __restrict__ p32,
                                                       uint32_t* __restrict__ bucket_cnt,
                                                       const unsigned long long* __restrict__ bucket_off,
                                                       uint32_t* __restrict__ entries,
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(1024) void bucket_kernel(const BucketTask* __restri
         const uint32_t t = s_task;
         if (t >= ntask) break;  // uniform: every thread read the same s_task
         const BucketTask T = tasks[t];
-        bucket_task<FILL>(T, primes, bucket_cnt, bucket_off, entries, overflow, log2w, cnt, lim);
+        bucket_task<FILL>(T, primes, p32, bucket_cnt, bucket_off, entries, overflow, log2w, cnt, lim);
     }
 }
 
@@ -293,22 +294,22 @@ hipError_t launch_stitch(const SieveArgs& a, hipStream_t s) {
 namespace voss {
 
 hipError_t launch_bucket_count(const BucketTask* tasks, uint32_t ntask, const PrimeRec* primes,
-                               uint32_t* bucket_cnt, uint32_t* task_next, int log2w, uint32_t grid,
-                               hipStream_t s) {
+                               const uint32_t* p32, uint32_t* bucket_cnt, uint32_t* task_next,
+                               int log2w, uint32_t grid, hipStream_t s) {
     if (ntask == 0) return hipSuccess;
     if (grid > ntask) grid = ntask;
-    hipLaunchKernelGGL(bucket_kernel<false>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes,
+    hipLaunchKernelGGL(bucket_kernel<false>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes, p32,
                        bucket_cnt, nullptr, nullptr, nullptr, task_next, log2w);
     return hipGetLastError();
 }
 
 hipError_t launch_bucket_fill(const BucketTask* tasks, uint32_t ntask, const PrimeRec* primes,
-                              const unsigned long long* bucket_off, uint32_t* bucket_cursor,
-                              uint32_t* entries, uint32_t* overflow, uint32_t* task_next, int log2w,
-                              uint32_t grid, hipStream_t s) {
+                              const uint32_t* p32, const unsigned long long* bucket_off,
+                              uint32_t* bucket_cursor, uint32_t* entries, uint32_t* overflow,
+                              uint32_t* task_next, int log2w, uint32_t grid, hipStream_t s) {
     if (ntask == 0) return hipSuccess;
     if (grid > ntask) grid = ntask;
-    hipLaunchKernelGGL(bucket_kernel<true>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes,
+    hipLaunchKernelGGL(bucket_kernel<true>, dim3(grid), dim3(1024), 0, s, tasks, ntask, primes, p32,
                        bucket_cursor, bucket_off, entries, overflow, task_next, log2w);
     return hipGetLastError();
 }

@@ -817,31 +817,62 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
 //               residue classes in turn (first multiple of the class, then
 //               stride p bytes).  Best when p is below the tile length
 //               (several hits per class);
-//   walk mode   thread per prime: one Barrett reduction gives the first
-//               multiple at or after the tile start, then the thread steps
+//   walk mode   thread per prime: one remainder (tile_mod, or mod_u64)
+//               gives the first multiple at or after the tile start, then
+//               the thread steps
 //               m -> m+1 (p/30 bytes and p%30 in the residue) and keeps the
 //               multiples coprime to 30.  Best for p beyond the tile length,
 //               where a class mode thread mostly finds nothing.
 // COUNT writes the hits per window to bucket_cnt.  FILL starts each
-// window's LDS cursor at its bucket's first slot and writes the entries
+// window's LDS cursor at its bucket's first slot (relative to the task's
+// first bucket) and writes the entries
 // (offset << 3) | bit; a bucket holds bucket_off[b+1] - bucket_off[b]
 // entries: hits beyond that are not written, the window's count (written to
 // bucket_cnt) still includes them and *overflow is set (the caller then
 // re-runs with exact counts), so no write ever leaves the bucket.
 
+// The first number X = 30*w0 of a bucket tile, for walk mode's X mod p in
+// double precision (tile_mod): xd = X rounded to double, delta = X - xd.
+// Usable (fp) when X < 2^63 and every walked prime is below 2^30.
+struct TileStart {
+    uint64_t X;
+    double xd;
+    int32_t delta;
+    bool fp;
+};
+
+__device__ __forceinline__ TileStart tile_start(uint64_t w0, uint32_t pmax) {
+    TileStart t;
+    t.X = 30ull * w0;
+    t.fp = t.X < (1ull << 63) && pmax < (1u << 30);
+    t.xd = (double)t.X;
+    t.delta = t.fp ? (int32_t)(int64_t)(t.X - (uint64_t)t.xd) : 0;
+    return t;
+}
+
+// X mod p for t.fp tiles without 64-bit integer multiplies: 1/p from the
+// hardware reciprocal and two Newton steps (relative error ~2^-52), the
+// quotient estimate q = floor(xd/p) (within 1 of X/p, since q < 2^46), then
+// X - q*p = fma(-q, p, xd) + delta, which is exact (an integer below 2^31
+// in magnitude), brought into [0, p) by one correction each way.
+__device__ __forceinline__ uint32_t tile_mod(const TileStart& t, uint32_t p) {
+    const double pd = (double)p;
+    double inv = __builtin_amdgcn_rcp(pd);
+    inv = __builtin_fma(__builtin_fma(-pd, inv, 1.0), inv, inv);
+    inv = __builtin_fma(__builtin_fma(-pd, inv, 1.0), inv, inv);
+    const double q = __builtin_floor(t.xd * inv);
+    int32_t r = (int32_t)__builtin_fma(-q, pd, t.xd) + t.delta;
+    r = r < 0 ? r + (int32_t)p : r;
+    r = r >= (int32_t)p ? r - (int32_t)p : r;
+    return (uint32_t)r;
+}
+
 // Walk mode: calls f(byte offset in the tile, wheel bit) for every hit of
-// prime pr in the tile starting at byte w0.
+// prime p in the tile of span bytes starting at number X; r = X mod p.
 template <typename F>
-__device__ __forceinline__ void walk_hits(uint64_t w0, const PrimeRec& pr, uint32_t span, F&& f) {
-    const uint32_t p = pr.p;
-    const uint64_t X = 30ull * w0;  // first number of the tile
-    uint32_t x;                     // first multiple m*p >= X with m >= 2, minus X
-    if (X < 2ull * p) {
-        x = (uint32_t)(2ull * p - X);
-    } else {
-        const uint32_t r = mod_u64(X, p, pr.magic);
-        x = r ? p - r : 0u;
-    }
+__device__ __forceinline__ void walk_hits(uint64_t X, uint32_t p, uint32_t r, uint32_t span, F&& f) {
+    // first multiple m*p >= X with m >= 2, minus X
+    const uint32_t x = X < 2ull * p ? (uint32_t)(2ull * p - X) : (r ? p - r : 0u);
     // the tile holds 30*span < 2^30 numbers; most walk-mode primes of a
     // short tile have no multiple in it, and one beyond 30*span at most one
     const uint32_t L = 30u * span;
@@ -931,6 +962,45 @@ __device__ __forceinline__ void bucket_sweep_primes(uint32_t i0, uint32_t i1,
     }
 }
 
+// g(p) for every prime of [i0, i1) from the packed 32-bit table, NPF per
+// thread at a time.
+template <int NPF, typename G>
+__device__ __forceinline__ void bucket_sweep_p32(uint32_t i0, uint32_t i1,
+                                                 const uint32_t* __restrict__ p32, G&& g) {
+    if (i1 <= i0) return;
+    const uint32_t n = i1 - i0;
+    const uint32_t last = i1 - 1u;
+    for (uint32_t k0 = threadIdx.x; k0 < n; k0 += (uint32_t)NPF * blockDim.x) {
+        uint32_t pv[NPF];
+#pragma unroll
+        for (int q = 0; q < NPF; q++) {
+            const uint32_t i = i0 + k0 + q * blockDim.x;
+            pv[q] = p32[i < last ? i : last];
+        }
+#pragma unroll
+        for (int q = 0; q < NPF; q++)
+            if (k0 + q * blockDim.x < n) g(pv[q]);
+    }
+}
+
+// Walk mode over the primes [i0, i1) of a tile: from the 4-byte packed
+// table with tile_mod where it applies (a third of the record bytes per
+// prime and no 64-bit multiplies), else from the records with mod_u64.
+template <int NPF, typename F>
+__device__ __forceinline__ void walk_sweep(const TileStart& t, uint32_t i0, uint32_t i1,
+                                           const PrimeRec* __restrict__ primes,
+                                           const uint32_t* __restrict__ p32, uint32_t span, F&& f) {
+    if (t.fp) {
+        bucket_sweep_p32<2 * NPF>(i0, i1, p32, [&](uint32_t p) {
+            walk_hits(t.X, p, tile_mod(t, p), span, f);
+        });
+    } else {
+        bucket_sweep_primes<NPF>(i0, i1, primes, [&](const PrimeRec& pr) {
+            walk_hits(t.X, pr.p, mod_u64(t.X, pr.p, pr.magic), span, f);
+        });
+    }
+}
+
 // One bucket task run by the whole
 // workgroup; cnt and lim are LDS arrays of kBucketTileMax entries.  Every
 // thread calls it; it ends with a barrier, so cnt and lim may be reused.
@@ -940,6 +1010,7 @@ __device__ __forceinline__ void bucket_sweep_primes(uint32_t i0, uint32_t i1,
 // occupancy is low and the record loads' latency shows).
 template <bool FILL, bool HOST_OFFSETS = false, int NPF = 4>
 __device__ __forceinline__ void bucket_task(const BucketTask& T, const PrimeRec* __restrict__ primes,
+                                            const uint32_t* __restrict__ p32,
                                             uint32_t* __restrict__ bucket_cnt,
                                             const unsigned long long* __restrict__ bucket_off_p,
                                             uint32_t* __restrict__ entries,
@@ -960,10 +1031,15 @@ __device__ __forceinline__ void bucket_task(const BucketTask& T, const PrimeRec*
     const uint32_t wmask = (1u << log2w) - 1u;
     const uint32_t span = T.nwin << log2w;  // bytes; at most 2^25
     auto count = [&](uint32_t b, uint32_t) { atomicAdd(&cnt[b >> log2w], 1u); };
+    const TileStart ts = tile_start(T.w0, T.i1 > T.iw ? p32[T.i1 - 1u] : 0u);
+    // slots count from the task's first bucket, so the LDS cursors stay
+    // 32-bit however many entries the round holds
+    const unsigned long long ebase = FILL ? bucket_off[T.b0] : 0ull;
+    uint32_t* const ent = FILL ? entries + ebase : nullptr;
     for (uint32_t i = tid; i < T.nwin; i += blockDim.x) {
         if (FILL && !T.shared) {
-            cnt[i] = (uint32_t)bucket_off[T.b0 + i];
-            lim[i] = (uint32_t)bucket_off[T.b0 + i + 1];
+            cnt[i] = (uint32_t)(bucket_off[T.b0 + i] - ebase);
+            lim[i] = (uint32_t)(bucket_off[T.b0 + i + 1] - ebase);
         } else {
             cnt[i] = 0;
         }
@@ -974,16 +1050,14 @@ __device__ __forceinline__ void bucket_task(const BucketTask& T, const PrimeRec*
         bucket_sweep_primes<NPF>(T.i0, T.iw, primes, [&](const PrimeRec& pr) {
             class_hits(T.w0, pr, span, count);
         });
-        bucket_sweep_primes<NPF>(T.iw, T.i1, primes, [&](const PrimeRec& pr) {
-            walk_hits(T.w0, pr, span, count);
-        });
+        walk_sweep<NPF>(ts, T.iw, T.i1, primes, p32, span, count);
         __syncthreads();
         for (uint32_t i = tid; i < T.nwin; i += blockDim.x) {
             const unsigned long long o0 = bucket_off[T.b0 + i], o1 = bucket_off[T.b0 + i + 1];
             const uint32_t start = cnt[i] ? atomicAdd(&bucket_cnt[T.b0 + i], cnt[i]) : 0u;
             if ((unsigned long long)start + cnt[i] > o1 - o0) atomicOr(overflow, 1u);
-            cnt[i] = (uint32_t)o0 + start;
-            lim[i] = (uint32_t)o1;
+            cnt[i] = (uint32_t)(o0 - ebase) + start;
+            lim[i] = (uint32_t)(o1 - ebase);
         }
         __syncthreads();
     }
@@ -991,19 +1065,15 @@ __device__ __forceinline__ void bucket_task(const BucketTask& T, const PrimeRec*
         bucket_sweep_primes<NPF>(T.i0, T.iw, primes, [&](const PrimeRec& pr) {
             class_hits(T.w0, pr, span, count);
         });
-        bucket_sweep_primes<NPF>(T.iw, T.i1, primes, [&](const PrimeRec& pr) {
-            walk_hits(T.w0, pr, span, count);
-        });
+        walk_sweep<NPF>(ts, T.iw, T.i1, primes, p32, span, count);
     } else {
         bucket_sweep_primes<NPF>(T.i0, T.iw, primes, [&](const PrimeRec& pr) {
-            bucket_fill_class(T.w0, pr, span, cnt, lim, entries, log2w, wmask);
+            bucket_fill_class(T.w0, pr, span, cnt, lim, ent, log2w, wmask);
         });
-        bucket_sweep_primes<NPF>(T.iw, T.i1, primes, [&](const PrimeRec& pr) {
-            walk_hits(T.w0, pr, span, [&](uint32_t b, uint32_t bit) {
-                const uint32_t w = b >> log2w;
-                const uint32_t slot = atomicAdd(&cnt[w], 1u);
-                if (slot < lim[w]) entries[slot] = ((b & wmask) << 3) | bit;
-            });
+        walk_sweep<NPF>(ts, T.iw, T.i1, primes, p32, span, [&](uint32_t b, uint32_t bit) {
+            const uint32_t w = b >> log2w;
+            const uint32_t slot = atomicAdd(&cnt[w], 1u);
+            if (slot < lim[w]) ent[slot] = ((b & wmask) << 3) | bit;
         });
     }
     __syncthreads();
@@ -1011,7 +1081,7 @@ __device__ __forceinline__ void bucket_task(const BucketTask& T, const PrimeRec*
         if (T.shared) {
             if (!FILL && cnt[i]) atomicAdd(&bucket_cnt[T.b0 + i], cnt[i]);
         } else if (FILL) {
-            bucket_cnt[T.b0 + i] = cnt[i] - (uint32_t)bucket_off[T.b0 + i];
+            bucket_cnt[T.b0 + i] = cnt[i] - (uint32_t)(bucket_off[T.b0 + i] - ebase);
             if (cnt[i] > lim[i]) atomicOr(overflow, 1u);
         } else {
             bucket_cnt[T.b0 + i] = cnt[i];

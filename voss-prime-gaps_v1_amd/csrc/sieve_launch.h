@@ -13,6 +13,7 @@ struct SieveArgs {
     uint32_t tier_a_end;    // first prime index marked one-lane-per-prime
     uint32_t restore0;      // word 0 bits of the presieved primes 7..73
     const PrimeRec* primes; // base primes >= 7, ascending
+    const uint32_t* p32;    // the same primes as 32-bit values
     const uint32_t* pat;                 // all tables, kPresieveWordsPadded words
     uint32_t pat_off[kPresieveGroups];   // table start in pat
     uint32_t pat_len[kPresieveGroups];   // period in words
@@ -50,11 +51,11 @@ hipError_t launch_sieve_emit(const SieveArgs& a, int window_bytes, uint32_t grid
 // (nbuckets + 1 offsets) and sets bucket_cursor[b] to the hits of window b;
 // a bucket that would exceed its room sets *overflow instead.
 hipError_t launch_bucket_count(const BucketTask* tasks, uint32_t ntask,
-                               const PrimeRec* primes, uint32_t* bucket_cnt,
+                               const PrimeRec* primes, const uint32_t* p32, uint32_t* bucket_cnt,
                                uint32_t* task_next, int log2w, uint32_t grid,
                                hipStream_t s);
 hipError_t launch_bucket_fill(const BucketTask* tasks, uint32_t ntask,
-                              const PrimeRec* primes,
+                              const PrimeRec* primes, const uint32_t* p32,
                               const unsigned long long* bucket_off,
                               uint32_t* bucket_cursor, uint32_t* entries,
                               uint32_t* overflow, uint32_t* task_next, int log2w,
