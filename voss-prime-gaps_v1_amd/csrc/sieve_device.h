@@ -298,7 +298,7 @@ struct alignas(16) SieveShared {
     uint32_t bm[WIN / 4];
     uint32_t hist[kHistBins];
     // q=30 rows interleaved: mq[8*(g/2) + row] for gaps g < 100; bin 0 takes
-    // the walk's gap-0 self-pairs and bin 50 its gaps >= 100 (both dropped)
+    // the walk's gap-0 self-pairs (dropped at the flush); bins 50.. unused
     uint32_t mq[64 * 8];
     // walk: how many nonzero wheel bytes of each value were seen (their
     // internal prime pairs go into mq at the flush)
