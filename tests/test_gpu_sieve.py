@@ -596,6 +596,28 @@ def test_interval_at_the_2_62_bound():
     assert_same(got, want)
 
 
+def test_interval_below_2_60_double_precision_remainders():
+    """[2^60 - 8e6, 2^60): three LDS windows whose bucket primes (2^20 up to
+    just below 2^30) take the bucket fill's double-precision remainder at its
+    largest quotient (X/p up to 2^40 per tile); expected outputs from the
+    interval's primes as the GPU Miller-Rabin finds them
+    (tests/from_primes.py)."""
+    import vossgpu
+    from from_primes import interval_results
+    lo, hi = 2**60 - 8 * 10**6, 2**60
+    other = vossgpu.Engine(0)
+    try:
+        ns = np.uint64(lo + 1) + np.arange(0, (hi - lo) // 2, dtype=np.uint64) * np.uint64(2)
+        print("[gpu] primes of the interval below 2^60 by Miller-Rabin", flush=True)
+        want = interval_results(ns[other.mr(ns).astype(bool)])
+        print("[gpu] interval below 2^60 (bucket primes below 2^30)", flush=True)
+        got = other.run_interval(lo, hi, large_cap=4096)
+    finally:
+        other.close()
+    assert want["total_primes"] > 150000
+    assert_same(got, want)
+
+
 def test_batch_of_high_intervals_matches_primality():
     """A batch of 12 intervals between 1e17 and 2^62 (six of 1e5 numbers, six
     of 8e6: three LDS windows each, so their primes from 2^20 to 2^31 go
