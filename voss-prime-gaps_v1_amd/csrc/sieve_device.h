@@ -88,6 +88,43 @@ __device__ __forceinline__ uint32_t mod_u64(uint64_t x, uint32_t p,
     return (uint32_t)(r >= p ? r - p : r);
 }
 
+// X mod p in double precision for many p and one X (a window or tile
+// start; start_mod): xd = X rounded to double, delta = X - xd.  Usable (fp)
+// when X < 2^63, every p is below 2^30 and every X/p below 2^50.
+struct ModStart {
+    uint64_t X;
+    double xd;
+    int32_t delta;
+    bool fp;
+};
+
+__device__ __forceinline__ ModStart mod_start(uint64_t X, uint32_t pmin, uint32_t pmax) {
+    ModStart t;
+    t.X = X;
+    t.fp = X < (1ull << 63) && pmax < (1u << 30) && (X >> 50) < pmin;
+    t.xd = (double)X;
+    t.delta = t.fp ? (int32_t)(int64_t)(X - (uint64_t)t.xd) : 0;
+    return t;
+}
+
+// X mod p for t.fp starts without 64-bit integer multiplies: 1/p from the
+// hardware reciprocal and two Newton steps (relative error ~2^-53), the
+// quotient estimate q = floor(xd/p) (xd*inv is within 2^-51 relative, so
+// within 1/2 of X/p < 2^50, and q within 1 of the quotient), then
+// X - q*p = fma(-q, p, xd) + delta, which is exact (an integer below 2^31
+// in magnitude), brought into [0, p) by one correction each way.
+__device__ __forceinline__ uint32_t start_mod(const ModStart& t, uint32_t p) {
+    const double pd = (double)p;
+    double inv = __builtin_amdgcn_rcp(pd);
+    inv = __builtin_fma(__builtin_fma(-pd, inv, 1.0), inv, inv);
+    inv = __builtin_fma(__builtin_fma(-pd, inv, 1.0), inv, inv);
+    const double q = __builtin_floor(t.xd * inv);
+    int32_t r = (int32_t)__builtin_fma(-q, pd, t.xd) + t.delta;
+    r = r < 0 ? r + (int32_t)p : r;
+    r = r >= (int32_t)p ? r - (int32_t)p : r;
+    return (uint32_t)r;
+}
+
 __device__ __forceinline__ void clear_bit(uint32_t* bm, uint32_t b,
                                           uint32_t bit) {
     atomicAnd(&bm[b >> 2], ~(1u << (((b & 3u) << 3) | bit)));
@@ -183,20 +220,14 @@ __device__ __forceinline__ uint32_t wheel_bit_of_residue(uint32_t r) {
 
 
 // Marks m*p for every m >= 2 coprime to 30 with m*p in the window
-// [30*w0, 30*(w0+len)): one Barrett reduction for the first multiple, then
+// [X, X + 30*len): from r = X mod p the first multiple, then
 // m -> m+1 steps (p/30 bytes and p%30 in the residue), keeping those
 // coprime to 30.  For p >= 30*len/7 this visits at most ~5 multiples, fewer
 // than the 8 residue-class starts of the class-by-class marking.
-__device__ __forceinline__ void walk_prime(uint32_t* bm, uint32_t p, unsigned long long magic,
-                                           uint64_t w0, uint32_t len) {
-    const uint64_t X = 30ull * w0;
-    uint32_t x;
-    if (X < 2ull * p) {
-        x = (uint32_t)(2ull * p - X);
-    } else {
-        const uint32_t r = mod_u64(X, p, magic);
-        x = r ? p - r : 0u;
-    }
+__device__ __forceinline__ void walk_prime(uint32_t* bm, uint32_t p, uint32_t r, uint64_t X,
+                                           uint32_t len) {
+    // r = X mod p, X = 30*w0 the window's first number
+    const uint32_t x = X < 2ull * p ? (uint32_t)(2ull * p - X) : (r ? p - r : 0u);
     uint32_t b = x / 30u, rr = x - 30u * b;
     const uint32_t pd = p / 30u, pm = p - 30u * pd;
     while (b < len) {
@@ -563,7 +594,10 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
                     if (pi0 + 64u * q >= np) continue;
                     const uint32_t p = pr[q].p;
                     if (7ull * p >= 30ull * len) {
-                        walk_prime(bm, p, pr[q].magic, D.w0, len);
+                        // (start_mod's double-precision remainder measured
+                        // slower here than the Barrett reduction)
+                        const uint64_t X = 30ull * D.w0;
+                        walk_prime(bm, p, mod_u64(X, p, pr[q].magic), X, len);
                     } else {
                         // at most one multiple per class in the window;
                         // pq*W30[j] by additions (W30 steps 6,4,2,4,2,4,6)
@@ -817,7 +851,7 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
 //               residue classes in turn (first multiple of the class, then
 //               stride p bytes).  Best when p is below the tile length
 //               (several hits per class);
-//   walk mode   thread per prime: one remainder (tile_mod, or mod_u64)
+//   walk mode   thread per prime: one remainder (start_mod, or mod_u64)
 //               gives the first multiple at or after the tile start, then
 //               the thread steps
 //               m -> m+1 (p/30 bytes and p%30 in the residue) and keeps the
@@ -830,42 +864,6 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
 // entries: hits beyond that are not written, the window's count (written to
 // bucket_cnt) still includes them and *overflow is set (the caller then
 // re-runs with exact counts), so no write ever leaves the bucket.
-
-// The first number X = 30*w0 of a bucket tile, for walk mode's X mod p in
-// double precision (tile_mod): xd = X rounded to double, delta = X - xd.
-// Usable (fp) when X < 2^63 and every walked prime is below 2^30.
-struct TileStart {
-    uint64_t X;
-    double xd;
-    int32_t delta;
-    bool fp;
-};
-
-__device__ __forceinline__ TileStart tile_start(uint64_t w0, uint32_t pmax) {
-    TileStart t;
-    t.X = 30ull * w0;
-    t.fp = t.X < (1ull << 63) && pmax < (1u << 30);
-    t.xd = (double)t.X;
-    t.delta = t.fp ? (int32_t)(int64_t)(t.X - (uint64_t)t.xd) : 0;
-    return t;
-}
-
-// X mod p for t.fp tiles without 64-bit integer multiplies: 1/p from the
-// hardware reciprocal and two Newton steps (relative error ~2^-52), the
-// quotient estimate q = floor(xd/p) (within 1 of X/p, since q < 2^46), then
-// X - q*p = fma(-q, p, xd) + delta, which is exact (an integer below 2^31
-// in magnitude), brought into [0, p) by one correction each way.
-__device__ __forceinline__ uint32_t tile_mod(const TileStart& t, uint32_t p) {
-    const double pd = (double)p;
-    double inv = __builtin_amdgcn_rcp(pd);
-    inv = __builtin_fma(__builtin_fma(-pd, inv, 1.0), inv, inv);
-    inv = __builtin_fma(__builtin_fma(-pd, inv, 1.0), inv, inv);
-    const double q = __builtin_floor(t.xd * inv);
-    int32_t r = (int32_t)__builtin_fma(-q, pd, t.xd) + t.delta;
-    r = r < 0 ? r + (int32_t)p : r;
-    r = r >= (int32_t)p ? r - (int32_t)p : r;
-    return (uint32_t)r;
-}
 
 // Walk mode: calls f(byte offset in the tile, wheel bit) for every hit of
 // prime p in the tile of span bytes starting at number X; r = X mod p.
@@ -983,16 +981,17 @@ __device__ __forceinline__ void bucket_sweep_p32(uint32_t i0, uint32_t i1,
     }
 }
 
-// Walk mode over the primes [i0, i1) of a tile: from the 4-byte packed
-// table with tile_mod where it applies (a third of the record bytes per
-// prime and no 64-bit multiplies), else from the records with mod_u64.
+// Walk mode over the primes [i0, i1) of a tile (t: its first number): from
+// the 4-byte packed table with start_mod where it applies (a third of the
+// record bytes per prime and no 64-bit multiplies), else from the records
+// with mod_u64.
 template <int NPF, typename F>
-__device__ __forceinline__ void walk_sweep(const TileStart& t, uint32_t i0, uint32_t i1,
+__device__ __forceinline__ void walk_sweep(const ModStart& t, uint32_t i0, uint32_t i1,
                                            const PrimeRec* __restrict__ primes,
                                            const uint32_t* __restrict__ p32, uint32_t span, F&& f) {
     if (t.fp) {
         bucket_sweep_p32<2 * NPF>(i0, i1, p32, [&](uint32_t p) {
-            walk_hits(t.X, p, tile_mod(t, p), span, f);
+            walk_hits(t.X, p, start_mod(t, p), span, f);
         });
     } else {
         bucket_sweep_primes<NPF>(i0, i1, primes, [&](const PrimeRec& pr) {
@@ -1031,7 +1030,8 @@ __device__ __forceinline__ void bucket_task(const BucketTask& T, const PrimeRec*
     const uint32_t wmask = (1u << log2w) - 1u;
     const uint32_t span = T.nwin << log2w;  // bytes; at most 2^25
     auto count = [&](uint32_t b, uint32_t) { atomicAdd(&cnt[b >> log2w], 1u); };
-    const TileStart ts = tile_start(T.w0, T.i1 > T.iw ? p32[T.i1 - 1u] : 0u);
+    const ModStart ts = mod_start(30ull * T.w0, T.i1 > T.iw ? p32[T.iw] : 1u,
+                                  T.i1 > T.iw ? p32[T.i1 - 1u] : 0u);
     // slots count from the task's first bucket, so the LDS cursors stay
     // 32-bit however many entries the round holds
     const unsigned long long ebase = FILL ? bucket_off[T.b0] : 0ull;
