@@ -53,6 +53,10 @@ struct ServeReq {
     size_t upub = 0;               // how many of them are published
     uint64_t nwin = 0, wpub = 0;   // an interval's windows, and how many are published
     std::vector<uint32_t> win_tile;      // tile of each window, ~0 without buckets
+    std::vector<uint32_t> tile_end;      // units (bucket tasks) up to and including each tile's
+    bool early = false;                  // a small request admitted onto an idle device: a tile's
+                                         // windows are published once its tasks are (they wait
+                                         // for them on the device)
     const uint32_t* tile_ready = nullptr;  // host flags the device sets per tile
     uint32_t bypassed = 0;         // later requests admitted while this one waited
 };
