@@ -596,6 +596,23 @@ def test_interval_at_the_2_62_bound():
     assert_same(got, want)
 
 
+def test_reciprocal_error_bound_of_the_tile_remainder():
+    """The bucket fill's double-precision remainder (start_mod) takes the
+    quotient within 1 of X/p < 2^47 only if 1/p (recip_newton: v_rcp_f64 and
+    one Newton step) is accurate to 2^-48 relative: checked on the device for
+    every odd p in [3, 2^30) (libvoss_probe.so)."""
+    import ctypes
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "voss-prime-gaps_v1_amd", "lib", "libvoss_probe.so")
+    L = ctypes.CDLL(path)
+    L.voss_probe_recip.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]
+    err = ctypes.c_double()
+    print("[gpu] recip_newton over the odd p below 2^30", flush=True)
+    assert L.voss_probe_recip(3, 1 << 30, ctypes.byref(err)) == 0
+    print("max relative error 2^%.2f" % math.log2(err.value), flush=True)
+    assert 0.0 < err.value < 2.0 ** -48, err.value
+
+
 def test_interval_below_2_60_double_precision_remainders():
     """[2^60 - 8e6, 2^60): three LDS windows whose bucket primes (2^20 up to
     just below 2^30) take the bucket fill's double-precision remainder at its

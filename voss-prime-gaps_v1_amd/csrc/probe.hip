@@ -1,12 +1,16 @@
 // probe.hip -- roofline microbenchmarks for gfx950 (used by bench.py to
 // state measured peaks beside the published ones): LDS bandwidth and
 // atomic rates, the HBM copy used to calibrate the traffic counters, and
-// the 64-bit Montgomery multiply rate that bounds Miller-Rabin.  The only
+// the 64-bit Montgomery multiply rate that bounds Miller-Rabin; and the
+// error bound of the double-precision reciprocal the bucket fill uses.  The only
 // global write of the compute probes is one word per thread, so the work
 // cannot be optimised away.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
+
+#include "fp_recip.h"
 #include "montgomery.h"
 
 namespace {
@@ -86,6 +90,20 @@ __global__ __launch_bounds__(256) void montmul_chains(uint32_t iters, unsigned l
     out[t] = x0 ^ x1 ^ x2 ^ x3;
 }
 
+// The largest relative error |1 - p*recip_newton(p)| over the odd p in
+// [lo, hi), as the bits of a non-negative double (they order like the
+// values), one atomic max per thread.
+__global__ void recip_error(uint32_t lo, uint32_t hi, unsigned long long* out) {
+    double m = 0.0;
+    const uint32_t n = (hi - lo) / 2u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double pd = (double)(lo + 2u * i);
+        const double e = __builtin_fabs(__builtin_fma(-pd, voss::recip_newton(pd), 1.0));
+        m = e > m ? e : m;
+    }
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
 }  // namespace
 
 // 64-bit Montgomery products per second on the whole device.
@@ -138,6 +156,24 @@ extern "C" int voss_probe_copy(size_t bytes, double* ms_out) {
     (void)hipFree(src);
     (void)hipFree(dst);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Max relative error of recip_newton over the odd p in [lo, hi) (lo odd).
+extern "C" int voss_probe_recip(uint32_t lo, uint32_t hi, double* max_err) {
+    if (!(lo & 1u) || hi <= lo) return -1;
+    unsigned long long* d = nullptr;
+    if (hipMalloc((void**)&d, 8) != hipSuccess) return -1;
+    unsigned long long h = 0;
+    int rc = -1;
+    if (hipMemcpy(d, &h, 8, hipMemcpyHostToDevice) == hipSuccess) {
+        hipLaunchKernelGGL(recip_error, dim3(4096), dim3(256), 0, 0, lo, hi, d);
+        if (hipDeviceSynchronize() == hipSuccess && hipMemcpy(&h, d, 8, hipMemcpyDeviceToHost) == hipSuccess) {
+            std::memcpy(max_err, &h, 8);
+            rc = 0;
+        }
+    }
+    (void)hipFree(d);
+    return rc;
 }
 
 // kind 0: read b128 -> bytes/s; 1: ds_and random -> ops/s; 2: ds_and linear.
