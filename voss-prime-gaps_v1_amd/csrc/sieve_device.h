@@ -3,8 +3,8 @@
 // and the persistent serving loop (serve_kernel.hip).
 //
 // One workgroup sieves one LDS window at a time (sieve_window):
-//   phase 0  the window's bitmap is initialised from 13 presieve pattern
-//            tables for primes 7..73, one AND per table per word;
+//   phase 0  the window's bitmap is initialised from 4 presieve pattern
+//            tables for primes 7..37, one AND per table per word;
 //   phase 1  the window's bucket hits (primes >= bucket_limit, precomputed in
 //            HBM) are applied, then every other base prime p with
 //            p*p <= window end crosses off its 8 residue classes with
@@ -321,7 +321,7 @@ __device__ __forceinline__ void pair_large(const PairCtx& c, uint32_t pb, uint32
 
 // LDS of one sieving workgroup.  A 128 KiB window keeps the presieve tables
 // in LDS (one workgroup per CU); smaller windows read them from global
-// memory (14.6 KB, L1 resident) so that two 64 KiB or four 32 KiB
+// memory (12.6 KB, L1 resident) so that two 64 KiB or four 32 KiB
 // workgroups fit in a CU's LDS.
 template <int WIN, int BLOCK, bool EMIT>
 struct alignas(16) SieveShared {
@@ -329,8 +329,11 @@ struct alignas(16) SieveShared {
     uint32_t bm[WIN / 4];
     uint32_t hist[kHistBins];
     // q=30 rows interleaved: mq[8*(g/2) + row] for gaps g < 100; bin 0 takes
-    // the walk's gap-0 self-pairs (dropped at the flush); bins 50.. unused
+    // the walk's gap-0 self-pairs (dropped at the flush); bins 50.. unused.
+    // The walk's odd lanes count into mq2, so two lanes of a wave that hit
+    // the same bin more often hit different words (summed at the flush)
     uint32_t mq[64 * 8];
+    uint32_t mq2[50 * 8];
     // walk: how many nonzero wheel bytes of each value were seen (their
     // internal prime pairs go into mq at the flush)
     uint32_t bytes[EMIT ? 1 : 256];
@@ -370,6 +373,7 @@ __device__ __forceinline__ void sieve_init_lds(const SieveArgs& a, SieveShared<W
     const int tid = threadIdx.x;
     for (int i = tid; i < kHistBins; i += BLOCK) S.hist[i] = 0;
     for (int i = tid; i < 8 * 64; i += BLOCK) S.mq[i] = 0;
+    for (int i = tid; i < 8 * 50; i += BLOCK) S.mq2[i] = 0;
     if constexpr (!EMIT) {
         for (int i = tid; i < 256; i += BLOCK) {
             S.bytes[i] = 0;
@@ -425,8 +429,9 @@ __device__ __forceinline__ void sieve_flush(SieveShared<WIN, BLOCK, EMIT>& S, Jo
         }
     }
     for (int i = tid; i < 8 * 64; i += BLOCK) {
-        const uint32_t v = S.mq[i];
+        const uint32_t v = S.mq[i] + (i < 8 * 50 ? S.mq2[i] : 0u);
         S.mq[i] = 0;
+        if (i < 8 * 50) S.mq2[i] = 0;
         if (v && i >= 8 && i < 8 * 50) {  // bins 1..49: gaps 2..98
             const uint32_t row = (uint32_t)i & 7u, h = (uint32_t)i >> 3;
             const unsigned long long v64 = v;
@@ -466,7 +471,7 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
     if (tid < kLgtCount && S.lgt[tid] >= 30ull * D.w0)
         atomicMin(&S.gmaybe, 2u * (uint32_t)(kLgtFirst + tid));
 
-    // ---- phase 0: presieve patterns (primes 7..73) from the tables
+    // ---- phase 0: presieve patterns (primes 7..37) from the tables
     {
         // each thread fills word pairs (2t, 2t+1), (2t+2B, 2t+2B+1), ...:
         // a pair reads two consecutive pattern words per table (the
@@ -761,7 +766,7 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
             // lanes with more primes do not hold the wave longer.  The
             // next word is loaded one iteration ahead.  Large-gap
             // candidates (g >= g_maybe, rare) branch to pair_large.
-            char* const mqb = (char*)S.mq;
+            char* const mqb = (lane & 1) ? (char*)S.mq2 : (char*)S.mq;
             char* const hb = (char*)S.hist;
             uint32_t wn = w;
             for (; i < iend; i++) {

@@ -23,17 +23,17 @@ constexpr int kModqRows = 10;         // q=6 {1,5}, q=30 {1,7,...,29}
 constexpr int kLargeMin = 101;        // merit only for g > 100 (:290)
 constexpr int kLargeAlways = 500;     // LARGE_GAP_THRESHOLD (:33)
 
-// Presieve: primes 7..73 are removed by ANDing period patterns held in
+// Presieve: primes 7..37 are removed by ANDing period patterns held in
 // LDS: one table of prod(p) words per group (the word pattern of a prime
-// p repeats every p words).  13 tables, 3652 words = 14.6 KB.
-constexpr int kPresieveGroups = 13;
-constexpr int kPresievePrimes = 18;   // 7,11,...,73
+// p repeats every p words).  4 tables, 3142 words = 12.6 KB.  Primes 41..73
+// had one table each; marked as tier A instead (a wave per prime) they cost
+// about what their table reads did and free 2 KB of LDS (1e11: presieve 13.1
+// -> 4.8 us per window, marking 67.2 -> 75.0 us).
+constexpr int kPresieveGroups = 4;
+constexpr int kPresievePrimes = 9;    // 7,11,...,37
 constexpr int kPresieveGroupPrimes[kPresieveGroups][3] = {
-    {7, 11, 13}, {17, 19, 1}, {23, 29, 1}, {31, 37, 1}, {41, 1, 1},
-    {43, 1, 1},  {47, 1, 1},  {53, 1, 1},  {59, 1, 1},  {61, 1, 1},
-    {67, 1, 1},  {71, 1, 1},  {73, 1, 1}};
-constexpr int kPresieveWords = 1001 + 323 + 667 + 1147 + 41 + 43 + 47 + 53 +
-                               59 + 61 + 67 + 71 + 73;
+    {7, 11, 13}, {17, 19, 1}, {23, 29, 1}, {31, 37, 1}};
+constexpr int kPresieveWords = 1001 + 323 + 667 + 1147;
 // Each table is followed by a copy of its first word, so two consecutive
 // pattern words can be read without a wrap check.
 constexpr int kPresieveWordsPadded = kPresieveWords + kPresieveGroups;
