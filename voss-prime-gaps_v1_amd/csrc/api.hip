@@ -865,10 +865,12 @@ int voss_autotune(voss_ctx* c, const uint64_t* lo, const uint64_t* hi, size_t n,
     double cur_s = 0;
     int rc = time_it(cur, &cur_s);
     if (rc) return rc;
-    // (window bytes, workgroups per CU): the LDS of one CU holds one 128 KiB
-    // window, two 64 KiB or four 32 KiB ones
+    // (window bytes, workgroups per CU): one CU holds one 128 KiB window's
+    // workgroup, two 64 KiB ones or three 32 KiB ones (their LDS: 162,880,
+    // 80,688 and 45,872 bytes of the CU's 163,840; the occupancy the runtime
+    // reports for the three sieve kernels)
     const std::vector<std::pair<uint32_t, uint32_t>> shapes = {
-        {131072, 1}, {65536, 2}, {65536, 1}, {32768, 4}};
+        {131072, 1}, {65536, 2}, {65536, 1}, {32768, 3}};
     const std::vector<uint32_t> tiers = {512, 768, 1024, 2048};
     const std::vector<uint32_t> blimits = {1u << 17, 1u << 18, 1u << 19, 1u << 20, 1u << 21,
                                            1u << 22, 1u << 23, 1u << 25, 0xFFFFFFFFu};
