@@ -417,6 +417,10 @@ SieveArgs make_args(voss_ctx* c, ExecBufs& B, uint32_t ndesc, unsigned long long
     a.tier_a_end = (uint32_t)(std::lower_bound(c->primes.begin(), c->primes.end(),
                                                c->tune.tier_a_limit) - c->primes.begin());
     if (a.tier_a_end < a.tier_a_begin) a.tier_a_end = a.tier_a_begin;
+    for (int h = 0; h < 3; h++)  // ceil(W / 4), ceil(W / 3), ceil(W / 2)
+        a.ib_frac[h] = (uint32_t)(std::lower_bound(c->primes.begin(), c->primes.end(),
+                                                   (c->tune.window_bytes + 3 - h) / (4 - h)) -
+                                  c->primes.begin());
     a.restore0 = c->restore0;
     a.primes = c->d_primes.p;
     a.p32 = c->d_p32.p;

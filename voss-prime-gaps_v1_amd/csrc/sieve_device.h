@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fp_recip.h"
+#include <type_traits>
 #include "sieve_launch.h"
 #include "wheel.h"
 
@@ -505,13 +506,23 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
         const uint32_t np = D.nprimes;
         const uint32_t a_end = np < a.tier_a_end ? np : a.tier_a_end;
         const uint32_t nA = a_end > a.tier_a_begin ? a_end - a.tier_a_begin : 0;
-        // tier B1: primes below len, 64 per task; tier B2: primes >= len
-        // (a few hits each), 256 per task with four record loads in
-        // flight per lane, so the loads' latency is paid once per task
+        // tier B1: primes below len/4 (below len in a partial window), 64
+        // per task, a loop per class; tier B2: primes >= len/4 (a few hits
+        // each), 256 per task with four record loads in flight per lane, so
+        // the loads' latency is paid once per task, and straight-line
+        // marking: in a full window [len/4, len/3), [len/3, len/2),
+        // [len/2, len) and [len, ...) have at most 4, 3, 2 and 1 hits per
+        // residue class, each range its own tasks
         const uint32_t b2 = max(a.tier_a_end, min(np, D.ibig));
-        const uint32_t nB1 = b2 > a.tier_a_end ? (b2 - a.tier_a_end + 63) / 64 : 0;
+        const bool full = D.len == (uint32_t)WIN;
+        const uint32_t c2 = full ? max(a.tier_a_end, min(b2, a.ib_frac[2])) : b2;
+        const uint32_t c3 = full ? max(a.tier_a_end, min(c2, a.ib_frac[1])) : b2;
+        const uint32_t c4 = full ? max(a.tier_a_end, min(c3, a.ib_frac[0])) : b2;
+        const uint32_t nB1 = c4 > a.tier_a_end ? (c4 - a.tier_a_end + 63) / 64 : 0;
+        const uint32_t n4 = (c3 - c4 + 255) / 256, n3 = (c2 - c3 + 255) / 256;
+        const uint32_t n2 = (b2 - c2 + 255) / 256;
         const uint32_t nB2 = np > b2 ? (np - b2 + 255) / 256 : 0;
-        const uint32_t ntask = nA + nB1 + nB2;
+        const uint32_t ntask = nA + nB1 + n4 + n3 + n2 + nB2;
         const uint32_t len = D.len;
         if (io.bent) {
             // large primes: their hits in this window, precomputed.
@@ -574,7 +585,7 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
                 for (; a4 < len; a4 += stride) and_at(bm, a4, m);
             } else if (t < nA + nB1) {
                 const uint32_t pi = a.tier_a_end + (t - nA) * 64 + lane;
-                if (pi < b2) {
+                if (pi < c4) {
                     const PrimeRec pr = a.primes[pi];
                     const uint32_t p = pr.p;
                     const uint32_t r = mod_u64(D.w0, p, pr.magic);
@@ -589,46 +600,61 @@ __device__ __forceinline__ void sieve_window(const SieveArgs& a, SieveShared<WIN
                     }
                 }
             } else {
-                const uint32_t pi0 = b2 + (t - nA - nB1) * 256 + lane;
-                PrimeRec pr[4];
+                // primes [first, end) of the range with at most HITS hits
+                // per class; task tt of the range
+                auto mark_few = [&](auto hits_c, uint32_t first, uint32_t end, uint32_t tt) {
+                    constexpr int HITS = decltype(hits_c)::value;
+                    const uint32_t pi0 = first + tt * 256 + lane;
+                    PrimeRec pr[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t pi = pi0 + 64u * q;
-                    pr[q] = a.primes[pi < np ? pi : np - 1u];
-                }
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t pi = pi0 + 64u * q;
+                        pr[q] = a.primes[pi < end ? pi : end - 1u];
+                    }
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    if (pi0 + 64u * q >= np) continue;
-                    const uint32_t p = pr[q].p;
-                    if (4ull * p >= 30ull * len) {
-                        // (start_mod's double-precision remainder measured
-                        // slower here than the Barrett reduction)
-                        const uint64_t X = 30ull * D.w0;
-                        walk_prime(bm, p, mod_u64(X, p, pr[q].magic), X, len);
-                    } else {
-                        // at most one multiple per class in the window;
-                        // pq*W30[j] by additions (W30 steps 6,4,2,4,2,4,6)
-                        const uint32_t r = mod_u64(D.w0, p, pr[q].magic);
-                        const uint32_t pq = p / 30u, pm = prime_mod30(pr[q].perm);
-                        uint32_t pq2 = pq + pq, pq4 = pq2 + pq2;
-                        // opaque to the compiler, which would otherwise
-                        // fold the additions back into multiplies
-                        asm volatile("" : "+v"(pq2), "+v"(pq4));
-                        const uint32_t pq6 = pq4 + pq2;
-                        uint32_t pqk = pq;
+                    for (int q = 0; q < 4; q++) {
+                        if (pi0 + 64u * q >= end) continue;
+                        const uint32_t p = pr[q].p;
+                        if (HITS == 1 && 4ull * p >= 30ull * len) {
+                            // (start_mod's double-precision remainder measured
+                            // slower here than the Barrett reduction)
+                            const uint64_t X = 30ull * D.w0;
+                            walk_prime(bm, p, mod_u64(X, p, pr[q].magic), X, len);
+                        } else {
+                            // at most HITS multiples per class in the window,
+                            // p bytes apart; pq*W30[j] by additions (W30
+                            // steps 6,4,2,4,2,4,6)
+                            const uint32_t r = mod_u64(D.w0, p, pr[q].magic);
+                            const uint32_t pq = p / 30u, pm = prime_mod30(pr[q].perm);
+                            uint32_t pq2 = pq + pq, pq4 = pq2 + pq2;
+                            // opaque to the compiler, which would otherwise
+                            // fold the additions back into multiplies
+                            asm volatile("" : "+v"(pq2), "+v"(pq4));
+                            const uint32_t pq6 = pq4 + pq2;
+                            uint32_t pqk = pq;
 #pragma unroll
-                        for (int j = 0; j < 8; j++) {
-                            const uint32_t k = w30_of((uint32_t)j);
-                            if (j > 0) {
-                                const uint32_t dk = k - w30_of((uint32_t)j - 1u);
-                                pqk += dk == 6u ? pq6 : dk == 4u ? pq4 : pq2;
+                            for (int j = 0; j < 8; j++) {
+                                const uint32_t k = w30_of((uint32_t)j);
+                                if (j > 0) {
+                                    const uint32_t dk = k - w30_of((uint32_t)j - 1u);
+                                    pqk += dk == 6u ? pq6 : dk == 4u ? pq4 : pq2;
+                                }
+                                const uint32_t b =
+                                    first_pos_off(class_off(pqk, pm, k), p, r, j == 0, D.w0);
+                                const uint32_t bit = (pr[q].perm >> (3 * j)) & 7u;
+                                if (b < len) clear_bit(bm, b, bit);
+                                if (HITS >= 2 && b + p < len) clear_bit(bm, b + p, bit);
+                                if (HITS >= 3 && b + 2u * p < len) clear_bit(bm, b + 2u * p, bit);
+                                if (HITS >= 4 && b + 3u * p < len) clear_bit(bm, b + 3u * p, bit);
                             }
-                            const uint32_t b =
-                                first_pos_off(class_off(pqk, pm, k), p, r, j == 0, D.w0);
-                            if (b < len) clear_bit(bm, b, (pr[q].perm >> (3 * j)) & 7u);
                         }
                     }
-                }
+                };
+                uint32_t u = t - nA - nB1;
+                if (u < n4) mark_few(std::integral_constant<int, 4>{}, c4, c3, u);
+                else if ((u -= n4) < n3) mark_few(std::integral_constant<int, 3>{}, c3, c2, u);
+                else if ((u -= n3) < n2) mark_few(std::integral_constant<int, 2>{}, c2, b2, u);
+                else mark_few(std::integral_constant<int, 1>{}, b2, np, u - n2);
             }
         }
     }

@@ -11,6 +11,8 @@ struct SieveArgs {
     uint32_t ndesc;
     uint32_t tier_a_begin;  // first prime index not covered by presieve
     uint32_t tier_a_end;    // first prime index marked one-lane-per-prime
+    uint32_t ib_frac[3];    // first prime indexes >= window_bytes/4, /3, /2
+                            // (tier B2's ranges in a full window)
     uint32_t restore0;      // word 0 bits of the presieved primes 7..37
     const PrimeRec* primes; // base primes >= 7, ascending
     const uint32_t* p32;    // the same primes as 32-bit values
