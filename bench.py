@@ -46,7 +46,7 @@ LDS_PEAK_GBS = 128 * 256 * 2.4e9 / 1e9
 # runs (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, scaled by a 1 GiB copy kernel);
 # the commands and raw counters are in
 # profiles/sieve_kernel_plain_1e11_hbm_traffic.md.
-SIEVE_1E11_HBM_BYTES = (2 * 3463.8 + 18654.6) * 1024
+SIEVE_1E11_HBM_BYTES = (2 * 3731.6 + 18697.3) * 1024
 
 
 def model_lds_bytes(N):
